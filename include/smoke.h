@@ -1,0 +1,225 @@
+/*
+ * smoke.h -- C ABI of the MI355X smoke time step (libsmokesim.so).
+ *
+ * Plain C: opaque handle, POD structs, pointers and sizes, status codes.  No
+ * HIP or C++ types cross this boundary.  Each entry point names the reference
+ * interface it replaces (project/smokeSimulation.cuh / .cu of
+ * RasmusAlmryd/smoke-simulation).  The reference's own nine C++ functions are
+ * kept, unchanged in name and meaning, by the header-only compat layer
+ * smokeSimulation.cuh in this directory, which calls the smoke_legacy_*
+ * entry points below.
+ *
+ * Layouts at the boundary are the reference's:
+ *   density, solid mask : nx*ny*nz, c = x + y*nx + z*nx*ny      (:267)
+ *   velocity component  : (nx+1)*(ny+1)*(nz+1),
+ *                         f = x + y*(nx+1) + z*(nx+1)*(ny+1)     (:146-147)
+ * Units are cells and seconds (the reference has no dx).
+ *
+ * Threading: calls on one handle are serialised by the library; different
+ * handles are independent.  Every call that returns smoke_status leaves a
+ * message for smoke_last_error() (per thread) when it fails.
+ */
+#ifndef SMOKE_H
+#define SMOKE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct smoke_sim smoke_sim;
+
+typedef enum smoke_status {
+    SMOKE_OK = 0,
+    SMOKE_ERR_INVALID = 1, /* bad argument (dims, id, size, NULL) */
+    SMOKE_ERR_HIP = 2,     /* HIP runtime error */
+    SMOKE_ERR_NOMEM = 3,   /* device allocation failed */
+    SMOKE_ERR_STATE = 4    /* call not valid in the current state */
+} smoke_status;
+
+/* Pressure projection.  REFERENCE is the reference's in-place red-black SOR on
+ * face velocities (smokeSimulation.cu:356-394, :795-801).  RBSOR and JACOBI
+ * solve the same linear system in pressure form (P, right-hand side div u*,
+ * then u -= grad P on fluid-fluid faces). */
+typedef enum smoke_projection {
+    SMOKE_PROJ_REFERENCE = 0,
+    SMOKE_PROJ_RBSOR = 1,
+    SMOKE_PROJ_JACOBI = 2
+} smoke_projection;
+
+/* How several obstacles combine into the interior mask.  LAST_WINS is the
+ * reference's fillObstacle (smokeSimulation.cu:295-311); UNION marks a cell
+ * solid if any obstacle covers it. */
+typedef enum smoke_obstacle_mode {
+    SMOKE_OBSTACLES_LAST_WINS = 0,
+    SMOKE_OBSTACLES_UNION = 1
+} smoke_obstacle_mode;
+
+/* Step parameters.  Defaults (smoke_params_default) are the reference's. */
+typedef struct smoke_params {
+    float gravity;      /* smokeSimulation.cu:28   -9.82; getGravity()        */
+    float buoyancy;     /* smokeSimulation.cu:29    2.0;  getBuoyancy()       */
+    float density_gate; /* buoyancy only where density > gate; <= 0: no gate  */
+    float max_velocity; /* smokeSimulation.cu:39    3.0 (speed clamp M)       */
+    double sor_omega;   /* smokeSimulation.cu:38    1.9                       */
+    double jacobi_omega;/* Jacobi damping, 1.0 = plain Jacobi                 */
+    int projection;     /* smoke_projection, default REFERENCE                */
+    int iterations;     /* smokeSimulation.cu:797  30: red-black iterations
+                           (two half-sweeps each) or Jacobi sweeps            */
+    int obstacle_mode;  /* smoke_obstacle_mode, default LAST_WINS             */
+    int warm_start;     /* pressure forms: start from the last step's P       */
+} smoke_params;
+
+/* Step stages, in execution order (for stage-level testing and timing). */
+typedef enum smoke_stage {
+    SMOKE_STAGE_SCENE = 0,       /* swap halves, sources, solid mask      */
+    SMOKE_STAGE_FORCE = 1,       /* buoyancy + speed clamp                */
+    SMOKE_STAGE_PROJECT = 2,     /* pressure projection                   */
+    SMOKE_STAGE_ADVECT_VEL = 3,  /* velocity advection [now] -> [past]    */
+    SMOKE_STAGE_ADVECT_DENS = 4, /* density advection  [now] -> [past]    */
+    SMOKE_STAGE_COUNT = 5
+} smoke_stage;
+
+/* ----------------------------------------------------------------- basics */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* smoke_last_error(void);
+
+/* Library version string. */
+const char* smoke_version(void);
+
+/* Replaces getGPUProperties (smokeSimulation.cu:62-85): writes a one-line
+ * description of device 0 into buf.  SMOKE_ERR_HIP if no GPU is visible. */
+smoke_status smoke_device_info(char* buf, size_t len);
+
+/* Reference defaults: gravity -9.82, buoyancy 2.0, no gate, M = 3, omega 1.9,
+ * REFERENCE projection, 30 iterations, LAST_WINS, cold start. */
+smoke_status smoke_params_default(smoke_params* out);
+
+/* ------------------------------------------------------------- lifecycle */
+
+/* Replaces initializeVolume (smokeSimulation.cu:124-238).  Allocates both
+ * ping-pong halves of density and u, v, w on the current device, copies
+ * `density` (nx*ny*nz floats, may be NULL for zeros) into half 0, zeroes the
+ * rest, and sets the solid mask to the y = 0 floor.  Requires n >= 3 per
+ * axis. */
+smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* density,
+                          smoke_sim** out);
+
+/* Replaces deleteVolume (smokeSimulation.cu:240-248).  NULL is a no-op. */
+void smoke_destroy(smoke_sim* sim);
+
+smoke_status smoke_get_dims(const smoke_sim* sim, unsigned* nx, unsigned* ny, unsigned* nz);
+
+/* Device bytes held by one simulation of the given size (budgeting). */
+size_t smoke_footprint_bytes(unsigned nx, unsigned ny, unsigned nz);
+
+/* Replaces the globals behind getGravity/getBuoyancy (smokeSimulation.cu:28-36)
+ * and the compile-time constants (:38-39, :797). */
+smoke_status smoke_set_params(smoke_sim* sim, const smoke_params* p);
+smoke_status smoke_get_params(const smoke_sim* sim, smoke_params* out);
+
+/* ---------------------------------------------------------------- scene */
+
+/* Replace addObstacle / addSmokeSource (smokeSimulation.cu:88-104).  Ids are
+ * shared by both kinds and count up from 0.  At most SMOKE_MAX_OBJECTS of each
+ * kind (the reference silently overflows its device buffers beyond 3). */
+#define SMOKE_MAX_OBJECTS 64
+smoke_status smoke_add_obstacle(smoke_sim* sim, float x, float y, float z, float vx, float vy,
+                                float vz, float r, int* id);
+smoke_status smoke_add_source(smoke_sim* sim, float x, float y, float z, float r, int* id);
+
+/* Replaces updateObjectPos (smokeSimulation.cu:106-109), with a bounds
+ * check. */
+smoke_status smoke_move_object(smoke_sim* sim, int id, float x, float y, float z);
+
+/* Base solid mask, nx*ny*nz bytes, nonzero = solid.  Obstacles are combined
+ * with it each step (reference: the mask is the y = 0 floor only,
+ * smokeSimulation.cu:200-207, which is also the initial value here). */
+smoke_status smoke_set_solids(smoke_sim* sim, const uint8_t* solid);
+
+/* ---------------------------------------------------------------- fields */
+
+/* Density of half h (0 or 1): nx*ny*nz floats. */
+smoke_status smoke_set_density(smoke_sim* sim, int half, const float* src);
+smoke_status smoke_get_density(smoke_sim* sim, int half, float* dst);
+
+/* Velocity component comp (0 = u, 1 = v, 2 = w) of half h:
+ * (nx+1)*(ny+1)*(nz+1) floats. */
+smoke_status smoke_set_velocity(smoke_sim* sim, int comp, int half, const float* src);
+smoke_status smoke_get_velocity(smoke_sim* sim, int comp, int half, float* dst);
+
+/* Solid mask (1 = fluid, as the reference's dev_s) and cell flags
+ * (bit 0 self fluid, bits 1-6 fluid neighbour x-,x+,y-,y+,z-,z+, bit 7 cell
+ * relaxed by the projection), nx*ny*nz bytes each, as of the last scene
+ * stage. */
+smoke_status smoke_get_mask(smoke_sim* sim, uint8_t* dst);
+smoke_status smoke_get_flags(smoke_sim* sim, uint8_t* dst);
+
+/* Pressure of the last pressure-form projection, nx*ny*nz floats. */
+smoke_status smoke_get_pressure(smoke_sim* sim, float* dst);
+
+/* Ping-pong parity (indexNow, smokeSimulation.cu:707): after a step the
+ * output density is half 1 - index_now. */
+int smoke_index_now(const smoke_sim* sim);
+
+/* ------------------------------------------------------------------ step */
+
+/* Replaces simulate (smokeSimulation.cu:774-819): one full step with time
+ * step dt, then, if density_out is not NULL, a blocking copy of the new
+ * density (nx*ny*nz floats) into it. */
+smoke_status smoke_step(smoke_sim* sim, float dt, float* density_out);
+
+/* Enqueues one full step on the simulation's stream and returns. */
+smoke_status smoke_step_async(smoke_sim* sim, float dt);
+
+/* Enqueues one stage (smoke_stage) of a step; five stages in order equal one
+ * step. */
+smoke_status smoke_run_stage(smoke_sim* sim, int stage, float dt);
+
+/* Blocking copy of the density produced by the last step (half
+ * 1 - index_now). */
+smoke_status smoke_read_density(smoke_sim* sim, float* dst);
+
+/* Waits for all work enqueued on the simulation's stream. */
+smoke_status smoke_synchronize(smoke_sim* sim);
+
+/* ---------------------------------------------------------------- timing */
+
+/* Per-stage device timing with HIP events recorded on the simulation's
+ * stream.  While enabled, every step accumulates, per stage, the elapsed time
+ * and the number of kernel launches of that stage. */
+smoke_status smoke_timing_enable(smoke_sim* sim, int enable);
+/* ms[SMOKE_STAGE_COUNT], launches[SMOKE_STAGE_COUNT] totals since the last
+ * reset, and the number of timed steps. Synchronises the stream. */
+smoke_status smoke_timing_read(smoke_sim* sim, double* ms, long long* launches, long long* steps);
+smoke_status smoke_timing_reset(smoke_sim* sim);
+/* Time between the first and the last pressure-sweep launch of every timed
+ * step (the projection's sweep loop, the step's dominant kernel) and the
+ * number of sweep launches in it, totals since the last reset. */
+smoke_status smoke_timing_sweeps(smoke_sim* sim, double* ms, long long* launches);
+/* Step latencies (ms) of the last `cap` timed steps, oldest first; *count
+ * receives how many were written. */
+smoke_status smoke_timing_step_ms(smoke_sim* sim, double* ms, int cap, int* count);
+
+/* ------------------------------------------------------ legacy interface */
+/* Process-global single simulation behind the reference's nine functions
+ * (smokeSimulation.cuh:4-17).  Used by smokeSimulation.cuh; errors are
+ * returned here and turned into the reference's print-and-exit there. */
+smoke_status smoke_legacy_device_info(void);                                   /* getGPUProperties */
+smoke_status smoke_legacy_initialize(float* grid, unsigned w, unsigned h, unsigned d); /* initializeVolume */
+smoke_status smoke_legacy_delete(void);                                         /* deleteVolume */
+smoke_status smoke_legacy_simulate(float* grid, float dt);                      /* simulate */
+int smoke_legacy_add_obstacle(float x, float y, float z, float vx, float vy, float vz, float r); /* addObstacle */
+int smoke_legacy_add_source(float x, float y, float z, float r);                /* addSmokeSource */
+smoke_status smoke_legacy_update_object_pos(int id, float x, float y, float z); /* updateObjectPos */
+float* smoke_legacy_buoyancy_ptr(void);                                          /* getBuoyancy */
+float* smoke_legacy_gravity_ptr(void);                                           /* getGravity */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMOKE_H */

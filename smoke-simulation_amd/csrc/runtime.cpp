@@ -1,0 +1,791 @@
+// Host runtime of libsmokesim.so: the C ABI of include/smoke.h over HIP.
+//
+// One smoke_sim owns one device arena (every field of both ping-pong halves,
+// pressure, right-hand side, masks and a staging buffer), one non-blocking
+// stream at default priority, the object list and the step parameters.  A
+// step is a fixed sequence of kernel launches on that stream; nothing in it
+// allocates or synchronises unless the caller asks for the density back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "smoke_internal.h"
+
+using smk::Grid;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+smoke_status fail(smoke_status st, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(SMOKE_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+struct Object {
+    float pos[3];
+    float vel[3];
+    float r;
+    int type;  // 0 obstacle, 1 source (smokeSimulation.cu:50)
+};
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+Grid make_grid(unsigned nx, unsigned ny, unsigned nz)
+{
+    Grid g;
+    g.nx = (int)nx;
+    g.ny = (int)ny;
+    g.nz = (int)nz;
+    g.px = (int)((nx + 1 + 31) / 32 * 32);
+    g.py = (int)ny + 1;
+    g.plane = g.px * g.py;
+    g.elems = g.plane * ((int)nz + 1);
+    return g;
+}
+
+constexpr int kFloatFields = 11;  // rho0 rho1 u0 u1 v0 v1 w0 w1 P0 P1 d
+constexpr int kByteFields = 3;    // base s flags
+
+size_t arena_bytes(const Grid& g)
+{
+    size_t e = (size_t)g.elems;
+    size_t staging = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) * sizeof(float);
+    return kFloatFields * align256(e * sizeof(float)) + align256(staging) +
+           kByteFields * align256(e);
+}
+
+bool dims_ok(unsigned nx, unsigned ny, unsigned nz)
+{
+    if (nx < 3 || ny < 3 || nz < 3) return false;
+    // 32-bit element indices in the kernels.
+    double elems = (double)((nx + 1 + 31) / 32 * 32) * (ny + 1) * (nz + 1);
+    return elems < 2147483647.0;
+}
+
+smoke_status check_params(const smoke_params* p)
+{
+    if (!p) return fail(SMOKE_ERR_INVALID, "params: NULL");
+    if (p->projection < SMOKE_PROJ_REFERENCE || p->projection > SMOKE_PROJ_JACOBI)
+        return fail(SMOKE_ERR_INVALID, "params: unknown projection %d", p->projection);
+    if (p->obstacle_mode != SMOKE_OBSTACLES_LAST_WINS && p->obstacle_mode != SMOKE_OBSTACLES_UNION)
+        return fail(SMOKE_ERR_INVALID, "params: unknown obstacle_mode %d", p->obstacle_mode);
+    if (p->iterations < 0 || p->iterations > 1000000)
+        return fail(SMOKE_ERR_INVALID, "params: iterations %d out of range", p->iterations);
+    if (!std::isfinite(p->sor_omega) || !std::isfinite(p->jacobi_omega))
+        return fail(SMOKE_ERR_INVALID, "params: omega must be finite");
+    if (!std::isfinite(p->gravity) || !std::isfinite(p->buoyancy) || !std::isfinite(p->max_velocity) ||
+        !std::isfinite(p->density_gate))
+        return fail(SMOKE_ERR_INVALID, "params: physics parameters must be finite");
+    return SMOKE_OK;
+}
+
+}  // namespace
+
+struct smoke_sim {
+    std::mutex mu;
+    Grid g{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char* arena = nullptr;
+    float* rho[2] = {nullptr, nullptr};
+    float* u[2] = {nullptr, nullptr};
+    float* v[2] = {nullptr, nullptr};
+    float* w[2] = {nullptr, nullptr};
+    float* P[2] = {nullptr, nullptr};
+    float* d = nullptr;
+    float* staging = nullptr;
+    uint8_t* base = nullptr;
+    uint8_t* s = nullptr;
+    uint8_t* flags = nullptr;
+
+    smoke_params prm{};
+    std::vector<Object> objects;
+    int now = 1;   // indexNow (smokeSimulation.cu:707)
+    int pcur = 0;  // P buffer holding the latest pressure
+    bool mask_dirty = true;
+
+    // Timing: events at every stage boundary of a step, plus two around the
+    // projection's sweep launches (the step's dominant kernel).
+    bool timing = false;
+    struct StepEvents {
+        hipEvent_t ev[SMOKE_STAGE_COUNT + 1];
+        long long launches[SMOKE_STAGE_COUNT];
+        hipEvent_t sweep_ev[2];
+        long long sweeps;
+    };
+    StepEvents* cur = nullptr;  // events of the step being enqueued
+    std::vector<StepEvents> pending;
+    std::vector<hipEvent_t> event_pool;
+    double stage_ms[SMOKE_STAGE_COUNT] = {0};
+    long long stage_launches[SMOKE_STAGE_COUNT] = {0};
+    double sweep_ms = 0.0;
+    long long sweep_launches = 0;
+    long long timed_steps = 0;
+    std::deque<double> step_ms;
+};
+
+namespace {
+
+constexpr size_t kMaxStepHistory = 1 << 20;
+
+smk::SphereList sphere_list(const smoke_sim* s, int type)
+{
+    smk::SphereList l;
+    l.count = 0;
+    for (const Object& o : s->objects)
+        if (o.type == type) l.s[l.count++] = make_float4(o.pos[0], o.pos[1], o.pos[2], o.r);
+    return l;
+}
+
+int count_type(const smoke_sim* s, int type)
+{
+    int n = 0;
+    for (const Object& o : s->objects) n += (o.type == type);
+    return n;
+}
+
+// Rebuilds mask and flags from the base mask and the obstacles.  The mask is a
+// pure function of those two, so it is recomputed only when one changed; the
+// reference recomputes it every step (drawObjects, smokeSimulation.cu:766).
+hipError_t refresh_mask(smoke_sim* s, long long* launches)
+{
+    if (!s->mask_dirty) return hipSuccess;
+    smk::SphereList obst = sphere_list(s, 0);
+    hipError_t e = smk::build_mask(s->g, s->base, s->s, obst, s->prm.obstacle_mode, s->stream);
+    if (e != hipSuccess) return e;
+    e = smk::build_flags(s->g, s->s, s->flags, s->stream);
+    if (e != hipSuccess) return e;
+    s->mask_dirty = false;
+    if (launches) *launches += 2;
+    return hipSuccess;
+}
+
+hipError_t stage_scene(smoke_sim* s, long long* launches)
+{
+    s->now = 1 - s->now;  // smokeSimulation.cu:777-779
+    for (const Object& o : s->objects) {
+        if (o.type != 1) continue;
+        int launched = 0;
+        hipError_t e = smk::fill_source(s->g, s->rho[0], s->rho[1],
+                                        make_float4(o.pos[0], o.pos[1], o.pos[2], o.r), s->stream,
+                                        &launched);
+        if (e != hipSuccess) return e;
+        *launches += launched;
+    }
+    return refresh_mask(s, launches);
+}
+
+hipError_t stage_force(smoke_sim* s, float dt, long long* launches)
+{
+    smk::ForceArgs a;
+    a.gravity = s->prm.gravity;
+    a.buoyancy = s->prm.buoyancy;
+    a.dt = dt;
+    a.max_v2 = s->prm.max_velocity * s->prm.max_velocity;
+    a.gate = s->prm.density_gate;
+    a.gate_on = s->prm.density_gate > 0.0f;
+    const int n = s->now;
+    *launches += 1;
+    return smk::force(s->g, s->u[n], s->v[n], s->w[n], s->rho[n], s->flags, a, s->stream);
+}
+
+// Records the begin (0) or end (1) event of the sweep loop of a timed step.
+hipError_t sweep_mark(smoke_sim* s, int which, long long sweeps)
+{
+    if (!s->cur) return hipSuccess;
+    if (which == 1) s->cur->sweeps = sweeps;
+    return hipEventRecord(s->cur->sweep_ev[which], s->stream);
+}
+
+hipError_t stage_project(smoke_sim* s, long long* launches)
+{
+    const Grid& g = s->g;
+    const int n = s->now;
+    const smoke_params& p = s->prm;
+    hipError_t e = hipSuccess;
+    if (p.projection == SMOKE_PROJ_REFERENCE) {
+        if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
+        for (int it = 0; it < p.iterations; ++it)
+            for (int o = 0; o < 2; ++o) {
+                e = smk::project_reference_half(g, s->u[n], s->v[n], s->w[n], s->flags, o,
+                                                p.sor_omega, s->stream);
+                if (e != hipSuccess) return e;
+                *launches += 1;
+            }
+        return sweep_mark(s, 1, 2LL * p.iterations);
+    }
+    e = smk::divergence(g, s->u[n], s->v[n], s->w[n], s->flags, s->d, s->stream);
+    if (e != hipSuccess) return e;
+    *launches += 1;
+    if (p.projection == SMOKE_PROJ_JACOBI) {
+        const bool plain = p.jacobi_omega == 1.0;
+        bool zero_in = !p.warm_start;
+        if (p.iterations == 0 && zero_in) {  // P = 0: no correction
+            if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
+            return sweep_mark(s, 1, 0);
+        }
+        if (!zero_in) {
+            e = smk::mask_pressure(g, s->P[s->pcur], s->flags, s->stream);
+            if (e != hipSuccess) return e;
+            *launches += 1;
+        }
+        if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
+        for (int it = 0; it < p.iterations; ++it) {
+            int out = 1 - s->pcur;
+            e = smk::jacobi_sweep(g, s->P[s->pcur], s->P[out], s->d, s->flags, (float)p.jacobi_omega,
+                                  plain ? 1 : 0, zero_in ? 1 : 0, s->stream);
+            if (e != hipSuccess) return e;
+            *launches += 1;
+            s->pcur = out;
+            zero_in = false;
+        }
+        if ((e = sweep_mark(s, 1, p.iterations)) != hipSuccess) return e;
+    } else {
+        if (p.warm_start) e = smk::mask_pressure(g, s->P[s->pcur], s->flags, s->stream);
+        else e = smk::fill_f32(s->P[s->pcur], g.elems, 0.0f, s->stream);
+        if (e != hipSuccess) return e;
+        *launches += 1;
+        if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
+        for (int it = 0; it < p.iterations; ++it)
+            for (int o = 0; o < 2; ++o) {
+                e = smk::rbsor_half(g, s->P[s->pcur], s->d, s->flags, o, (float)p.sor_omega, s->stream);
+                if (e != hipSuccess) return e;
+                *launches += 1;
+            }
+        if ((e = sweep_mark(s, 1, 2LL * p.iterations)) != hipSuccess) return e;
+    }
+    e = smk::gradient(g, s->u[n], s->v[n], s->w[n], s->P[s->pcur], s->flags, s->stream);
+    *launches += 1;
+    return e;
+}
+
+hipError_t stage_advect_velocity(smoke_sim* s, float dt, long long* launches)
+{
+    const int n = s->now, p = 1 - s->now;
+    *launches += 1;
+    return smk::advect_velocity(s->g, s->u[n], s->v[n], s->w[n], s->u[p], s->v[p], s->w[p], s->flags,
+                                dt, s->stream);
+}
+
+hipError_t stage_advect_density(smoke_sim* s, float dt, long long* launches)
+{
+    const int n = s->now, p = 1 - s->now;
+    *launches += 1;
+    return smk::advect_density(s->g, s->rho[n], s->rho[p], s->u[p], s->v[p], s->w[p], s->flags, dt,
+                               s->stream);
+}
+
+hipError_t run_stage(smoke_sim* s, int stage, float dt, long long* launches)
+{
+    switch (stage) {
+    case SMOKE_STAGE_SCENE: return stage_scene(s, launches);
+    case SMOKE_STAGE_FORCE: return stage_force(s, dt, launches);
+    case SMOKE_STAGE_PROJECT: return stage_project(s, launches);
+    case SMOKE_STAGE_ADVECT_VEL: return stage_advect_velocity(s, dt, launches);
+    case SMOKE_STAGE_ADVECT_DENS: return stage_advect_density(s, dt, launches);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t get_event(smoke_sim* s, hipEvent_t* ev)
+{
+    if (!s->event_pool.empty()) {
+        *ev = s->event_pool.back();
+        s->event_pool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(ev);
+}
+
+// Folds finished timed steps into the totals (all of them if `wait`).
+hipError_t collect_timing(smoke_sim* s, bool wait)
+{
+    size_t done = 0;
+    for (; done < s->pending.size(); ++done) {
+        smoke_sim::StepEvents& se = s->pending[done];
+        if (wait) {
+            hipError_t e = hipEventSynchronize(se.ev[SMOKE_STAGE_COUNT]);
+            if (e != hipSuccess) return e;
+        } else if (hipEventQuery(se.ev[SMOKE_STAGE_COUNT]) != hipSuccess) {
+            break;
+        }
+        double total = 0.0;
+        for (int k = 0; k < SMOKE_STAGE_COUNT; ++k) {
+            float ms = 0.0f;
+            hipError_t e = hipEventElapsedTime(&ms, se.ev[k], se.ev[k + 1]);
+            if (e != hipSuccess) return e;
+            s->stage_ms[k] += ms;
+            s->stage_launches[k] += se.launches[k];
+            total += ms;
+        }
+        float sweep = 0.0f;
+        hipError_t e = hipEventElapsedTime(&sweep, se.sweep_ev[0], se.sweep_ev[1]);
+        if (e != hipSuccess) return e;
+        s->sweep_ms += sweep;
+        s->sweep_launches += se.sweeps;
+        s->timed_steps += 1;
+        s->step_ms.push_back(total);
+        if (s->step_ms.size() > kMaxStepHistory) s->step_ms.pop_front();
+        for (int k = 0; k <= SMOKE_STAGE_COUNT; ++k) s->event_pool.push_back(se.ev[k]);
+        s->event_pool.push_back(se.sweep_ev[0]);
+        s->event_pool.push_back(se.sweep_ev[1]);
+    }
+    s->pending.erase(s->pending.begin(), s->pending.begin() + (long)done);
+    return hipSuccess;
+}
+
+smoke_status step_locked(smoke_sim* s, float dt)
+{
+    if (!std::isfinite(dt)) return fail(SMOKE_ERR_INVALID, "step: dt must be finite");
+    smoke_sim::StepEvents se{};
+    if (s->timing) {
+        if (s->pending.size() >= 512) HIP_TRY(collect_timing(s, false));
+        for (int k = 0; k <= SMOKE_STAGE_COUNT; ++k) HIP_TRY(get_event(s, &se.ev[k]));
+        HIP_TRY(get_event(s, &se.sweep_ev[0]));
+        HIP_TRY(get_event(s, &se.sweep_ev[1]));
+        HIP_TRY(hipEventRecord(se.ev[0], s->stream));
+        s->cur = &se;
+    }
+    for (int st = 0; st < SMOKE_STAGE_COUNT; ++st) {
+        long long launches = 0;
+        hipError_t e = run_stage(s, st, dt, &launches);
+        if (e != hipSuccess) {
+            s->cur = nullptr;
+            return fail(SMOKE_ERR_HIP, "step: stage %d launch failed: %s", st, hipGetErrorString(e));
+        }
+        if (s->timing) {
+            se.launches[st] = launches;
+            HIP_TRY(hipEventRecord(se.ev[st + 1], s->stream));
+        }
+    }
+    if (s->timing) s->pending.push_back(se);
+    s->cur = nullptr;
+    return SMOKE_OK;
+}
+
+// Blocking device->host copy of a padded cell field through the staging
+// buffer.
+smoke_status read_cells_f32(smoke_sim* s, const float* src, float* dst)
+{
+    HIP_TRY(smk::pack_cells_f32(s->g, src, s->staging, s->stream));
+    size_t bytes = (size_t)s->g.nx * s->g.ny * s->g.nz * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char* smoke_last_error(void) { return g_last_error.c_str(); }
+
+const char* smoke_version(void) { return "smokesim 0.1 (gfx950, FP32)"; }
+
+smoke_status smoke_device_info(char* buf, size_t len)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(SMOKE_ERR_HIP, "no HIP device visible");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (buf && len)
+        snprintf(buf, len, "device %d: %s (%s), %d CUs, %.1f GiB, LDS/block %zu B, %d devices", dev,
+                 prop.name, prop.gcnArchName, prop.multiProcessorCount,
+                 prop.totalGlobalMem / 1073741824.0, prop.sharedMemPerBlock, count);
+    return SMOKE_OK;
+}
+
+smoke_status smoke_params_default(smoke_params* out)
+{
+    if (!out) return fail(SMOKE_ERR_INVALID, "params_default: NULL");
+    out->gravity = -9.82f;
+    out->buoyancy = 2.0f;
+    out->density_gate = 0.0f;
+    out->max_velocity = 3.0f;
+    out->sor_omega = 1.9;
+    out->jacobi_omega = 1.0;
+    out->projection = SMOKE_PROJ_REFERENCE;
+    out->iterations = 30;
+    out->obstacle_mode = SMOKE_OBSTACLES_LAST_WINS;
+    out->warm_start = 0;
+    return SMOKE_OK;
+}
+
+size_t smoke_footprint_bytes(unsigned nx, unsigned ny, unsigned nz)
+{
+    if (!dims_ok(nx, ny, nz)) return 0;
+    return arena_bytes(make_grid(nx, ny, nz));
+}
+
+smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* density,
+                          smoke_sim** out)
+{
+    if (!out) return fail(SMOKE_ERR_INVALID, "create: out is NULL");
+    *out = nullptr;
+    if (!dims_ok(nx, ny, nz))
+        return fail(SMOKE_ERR_INVALID, "create: dims %ux%ux%u invalid (each >= 3, padded size < 2^31)",
+                    nx, ny, nz);
+    smoke_sim* s = new smoke_sim();
+    s->g = make_grid(nx, ny, nz);
+    smoke_params_default(&s->prm);
+    auto bail = [&](smoke_status st) {
+        smoke_destroy(s);
+        return st;
+    };
+    hipError_t e = hipGetDevice(&s->device);
+    if (e != hipSuccess) return bail(fail(SMOKE_ERR_HIP, "create: no device: %s", hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(fail(SMOKE_ERR_HIP, "create: stream: %s", hipGetErrorString(e)));
+    size_t bytes = arena_bytes(s->g);
+    e = hipMalloc((void**)&s->arena, bytes);
+    if (e != hipSuccess) {
+        s->arena = nullptr;
+        return bail(fail(SMOKE_ERR_NOMEM, "create: cannot allocate %.2f GiB: %s", bytes / 1073741824.0,
+                         hipGetErrorString(e)));
+    }
+    const size_t fbytes = align256((size_t)s->g.elems * sizeof(float));
+    char* p = s->arena;
+    float** fields[kFloatFields] = {&s->rho[0], &s->rho[1], &s->u[0], &s->u[1], &s->v[0], &s->v[1],
+                                    &s->w[0],   &s->w[1],   &s->P[0], &s->P[1], &s->d};
+    for (int k = 0; k < kFloatFields; ++k) {
+        *fields[k] = reinterpret_cast<float*>(p);
+        p += fbytes;
+    }
+    s->staging = reinterpret_cast<float*>(p);
+    p += align256((size_t)(nx + 1) * (ny + 1) * (nz + 1) * sizeof(float));
+    uint8_t** bfields[kByteFields] = {&s->base, &s->s, &s->flags};
+    for (int k = 0; k < kByteFields; ++k) {
+        *bfields[k] = reinterpret_cast<uint8_t*>(p);
+        p += align256((size_t)s->g.elems);
+    }
+    // Zero everything (the reference leaves half [1] uninitialised; zero is
+    // the defined value here), then the floor mask and the initial density.
+    e = smk::fill_f32(s->rho[0], (long long)(kFloatFields * fbytes / sizeof(float)), 0.0f, s->stream);
+    if (e == hipSuccess) e = smk::fill_u8(s->base, (long long)(kByteFields * align256(s->g.elems)), 0, s->stream);
+    if (e == hipSuccess) e = smk::floor_mask(s->g, s->base, s->stream);
+    if (e == hipSuccess && density) {
+        size_t db = (size_t)nx * ny * nz * sizeof(float);
+        e = hipMemcpyAsync(s->staging, density, db, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = smk::unpack_cells_f32(s->g, s->staging, s->rho[0], s->stream);
+    }
+    if (e == hipSuccess) e = refresh_mask(s, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return bail(fail(SMOKE_ERR_HIP, "create: init failed: %s", hipGetErrorString(e)));
+    *out = s;
+    return SMOKE_OK;
+}
+
+void smoke_destroy(smoke_sim* s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& se : s->pending) {
+        for (int k = 0; k <= SMOKE_STAGE_COUNT; ++k) (void)hipEventDestroy(se.ev[k]);
+        (void)hipEventDestroy(se.sweep_ev[0]);
+        (void)hipEventDestroy(se.sweep_ev[1]);
+    }
+    for (hipEvent_t ev : s->event_pool) (void)hipEventDestroy(ev);
+    if (s->arena) (void)hipFree(s->arena);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+smoke_status smoke_get_dims(const smoke_sim* s, unsigned* nx, unsigned* ny, unsigned* nz)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "get_dims: NULL sim");
+    if (nx) *nx = (unsigned)s->g.nx;
+    if (ny) *ny = (unsigned)s->g.ny;
+    if (nz) *nz = (unsigned)s->g.nz;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "set_params: NULL sim");
+    smoke_status st = check_params(p);
+    if (st != SMOKE_OK) return st;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (p->obstacle_mode != s->prm.obstacle_mode) s->mask_dirty = true;
+    s->prm = *p;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_params(const smoke_sim* s, smoke_params* out)
+{
+    if (!s || !out) return fail(SMOKE_ERR_INVALID, "get_params: NULL");
+    *out = s->prm;
+    return SMOKE_OK;
+}
+
+static smoke_status add_object(smoke_sim* s, int type, float x, float y, float z, float vx, float vy,
+                               float vz, float r, int* id)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "add object: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (count_type(s, type) >= SMOKE_MAX_OBJECTS)
+        return fail(SMOKE_ERR_INVALID, "add object: more than %d %s", SMOKE_MAX_OBJECTS,
+                    type == 0 ? "obstacles" : "sources");
+    Object o;
+    o.pos[0] = x; o.pos[1] = y; o.pos[2] = z;
+    o.vel[0] = vx; o.vel[1] = vy; o.vel[2] = vz;
+    o.r = r;
+    o.type = type;
+    s->objects.push_back(o);
+    if (type == 0) s->mask_dirty = true;
+    if (id) *id = (int)s->objects.size() - 1;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_add_obstacle(smoke_sim* s, float x, float y, float z, float vx, float vy, float vz,
+                                float r, int* id)
+{
+    return add_object(s, 0, x, y, z, vx, vy, vz, r, id);
+}
+
+smoke_status smoke_add_source(smoke_sim* s, float x, float y, float z, float r, int* id)
+{
+    return add_object(s, 1, x, y, z, 0.0f, 0.0f, 0.0f, r, id);
+}
+
+smoke_status smoke_move_object(smoke_sim* s, int id, float x, float y, float z)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "move_object: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int)s->objects.size())
+        return fail(SMOKE_ERR_INVALID, "move_object: no object with id %d", id);
+    Object& o = s->objects[id];
+    o.pos[0] = x; o.pos[1] = y; o.pos[2] = z;
+    if (o.type == 0) s->mask_dirty = true;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_set_solids(smoke_sim* s, const uint8_t* solid)
+{
+    if (!s || !solid) return fail(SMOKE_ERR_INVALID, "set_solids: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    size_t n = (size_t)s->g.nx * s->g.ny * s->g.nz;
+    HIP_TRY(hipMemcpyAsync(s->staging, solid, n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(smk::unpack_cells_solid(s->g, reinterpret_cast<const uint8_t*>(s->staging), s->base, s->stream));
+    s->mask_dirty = true;
+    HIP_TRY(refresh_mask(s, nullptr));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_set_density(smoke_sim* s, int half, const float* src)
+{
+    if (!s || !src || (half != 0 && half != 1)) return fail(SMOKE_ERR_INVALID, "set_density: bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    size_t bytes = (size_t)s->g.nx * s->g.ny * s->g.nz * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(smk::unpack_cells_f32(s->g, s->staging, s->rho[half], s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_density(smoke_sim* s, int half, float* dst)
+{
+    if (!s || !dst || (half != 0 && half != 1)) return fail(SMOKE_ERR_INVALID, "get_density: bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return read_cells_f32(s, s->rho[half], dst);
+}
+
+static float* comp_ptr(smoke_sim* s, int comp, int half)
+{
+    if (half != 0 && half != 1) return nullptr;
+    switch (comp) {
+    case 0: return s->u[half];
+    case 1: return s->v[half];
+    case 2: return s->w[half];
+    default: return nullptr;
+    }
+}
+
+smoke_status smoke_set_velocity(smoke_sim* s, int comp, int half, const float* src)
+{
+    if (!s || !src) return fail(SMOKE_ERR_INVALID, "set_velocity: NULL");
+    float* dst = comp_ptr(s, comp, half);
+    if (!dst) return fail(SMOKE_ERR_INVALID, "set_velocity: bad component %d or half %d", comp, half);
+    std::lock_guard<std::mutex> lk(s->mu);
+    size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(smk::unpack_faces_f32(s->g, s->staging, dst, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_velocity(smoke_sim* s, int comp, int half, float* dst)
+{
+    if (!s || !dst) return fail(SMOKE_ERR_INVALID, "get_velocity: NULL");
+    float* src = comp_ptr(s, comp, half);
+    if (!src) return fail(SMOKE_ERR_INVALID, "get_velocity: bad component %d or half %d", comp, half);
+    std::lock_guard<std::mutex> lk(s->mu);
+    HIP_TRY(smk::pack_faces_f32(s->g, src, s->staging, s->stream));
+    size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+static smoke_status read_bytes(smoke_sim* s, const uint8_t* src, uint8_t* dst)
+{
+    uint8_t* st = reinterpret_cast<uint8_t*>(s->staging);
+    HIP_TRY(smk::pack_cells_u8(s->g, src, st, s->stream));
+    HIP_TRY(hipMemcpyAsync(dst, st, (size_t)s->g.nx * s->g.ny * s->g.nz, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_mask(smoke_sim* s, uint8_t* dst)
+{
+    if (!s || !dst) return fail(SMOKE_ERR_INVALID, "get_mask: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return read_bytes(s, s->s, dst);
+}
+
+smoke_status smoke_get_flags(smoke_sim* s, uint8_t* dst)
+{
+    if (!s || !dst) return fail(SMOKE_ERR_INVALID, "get_flags: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return read_bytes(s, s->flags, dst);
+}
+
+smoke_status smoke_get_pressure(smoke_sim* s, float* dst)
+{
+    if (!s || !dst) return fail(SMOKE_ERR_INVALID, "get_pressure: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return read_cells_f32(s, s->P[s->pcur], dst);
+}
+
+int smoke_index_now(const smoke_sim* s) { return s ? s->now : -1; }
+
+smoke_status smoke_step_async(smoke_sim* s, float dt)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "step: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return step_locked(s, dt);
+}
+
+smoke_status smoke_step(smoke_sim* s, float dt, float* density_out)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "step: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    smoke_status st = step_locked(s, dt);
+    if (st != SMOKE_OK) return st;
+    if (density_out) return read_cells_f32(s, s->rho[1 - s->now], density_out);
+    return SMOKE_OK;
+}
+
+smoke_status smoke_run_stage(smoke_sim* s, int stage, float dt)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "run_stage: NULL sim");
+    if (stage < 0 || stage >= SMOKE_STAGE_COUNT) return fail(SMOKE_ERR_INVALID, "run_stage: bad stage %d", stage);
+    if (!std::isfinite(dt)) return fail(SMOKE_ERR_INVALID, "run_stage: dt must be finite");
+    std::lock_guard<std::mutex> lk(s->mu);
+    long long launches = 0;
+    hipError_t e = run_stage(s, stage, dt, &launches);
+    if (e != hipSuccess) return fail(SMOKE_ERR_HIP, "run_stage %d: %s", stage, hipGetErrorString(e));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_read_density(smoke_sim* s, float* dst)
+{
+    if (!s || !dst) return fail(SMOKE_ERR_INVALID, "read_density: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    return read_cells_f32(s, s->rho[1 - s->now], dst);
+}
+
+smoke_status smoke_synchronize(smoke_sim* s)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "synchronize: NULL sim");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_timing_enable(smoke_sim* s, int enable)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "timing_enable: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->timing = enable != 0;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_timing_read(smoke_sim* s, double* ms, long long* launches, long long* steps)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "timing_read: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    HIP_TRY(collect_timing(s, true));
+    for (int k = 0; k < SMOKE_STAGE_COUNT; ++k) {
+        if (ms) ms[k] = s->stage_ms[k];
+        if (launches) launches[k] = s->stage_launches[k];
+    }
+    if (steps) *steps = s->timed_steps;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_timing_reset(smoke_sim* s)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "timing_reset: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    HIP_TRY(collect_timing(s, true));
+    for (int k = 0; k < SMOKE_STAGE_COUNT; ++k) {
+        s->stage_ms[k] = 0.0;
+        s->stage_launches[k] = 0;
+    }
+    s->sweep_ms = 0.0;
+    s->sweep_launches = 0;
+    s->timed_steps = 0;
+    s->step_ms.clear();
+    return SMOKE_OK;
+}
+
+smoke_status smoke_timing_sweeps(smoke_sim* s, double* ms, long long* launches)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "timing_sweeps: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    HIP_TRY(collect_timing(s, true));
+    if (ms) *ms = s->sweep_ms;
+    if (launches) *launches = s->sweep_launches;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_timing_step_ms(smoke_sim* s, double* ms, int cap, int* count)
+{
+    if (!s || (!ms && cap > 0)) return fail(SMOKE_ERR_INVALID, "timing_step_ms: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    HIP_TRY(collect_timing(s, true));
+    int n = (int)std::min<size_t>((size_t)std::max(cap, 0), s->step_ms.size());
+    size_t first = s->step_ms.size() - (size_t)n;
+    for (int i = 0; i < n; ++i) ms[i] = s->step_ms[first + (size_t)i];
+    if (count) *count = n;
+    return SMOKE_OK;
+}
+
+}  // extern "C"

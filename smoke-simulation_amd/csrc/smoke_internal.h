@@ -1,0 +1,102 @@
+// Internal declarations shared by the host runtime and the HIP kernels of
+// libsmokesim.so.  Not part of the public ABI (include/smoke.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "smoke.h"
+
+namespace smk {
+
+// One padded layout for every field.  Cell (x,y,z) and the faces on its low
+// x/y/z sides share the index x + px*y + plane*z, so a kernel addresses
+// density, mask, pressure and u/v/w with one index.  Rows are padded to a
+// multiple of 32 elements (128 B of FP32) so every row starts on a cache-line
+// boundary; planes hold ny+1 rows and there are nz+1 planes, the extents of
+// the reference's staggered arrays (smokeSimulation.cu:146-147).
+struct Grid {
+    int nx, ny, nz;
+    int px;     // row pitch in elements, >= nx+1, multiple of 32
+    int py;     // rows per plane, ny+1
+    int plane;  // px*py
+    int elems;  // plane*(nz+1)
+    __host__ __device__ __forceinline__ int idx(int x, int y, int z) const
+    {
+        return x + px * y + plane * z;
+    }
+};
+
+// Cell flag bits (one byte per cell).
+enum : uint8_t {
+    F_SELF = 1,
+    F_XM = 2,
+    F_XP = 4,
+    F_YM = 8,
+    F_YP = 16,
+    F_ZM = 32,
+    F_ZP = 64,
+    F_ACTIVE = 128,
+};
+constexpr uint8_t F_NBRS = F_XM | F_XP | F_YM | F_YP | F_ZM | F_ZP;
+
+// Spheres passed by value as a kernel argument: x, y, z, radius.
+struct SphereList {
+    int count;
+    float4 s[SMOKE_MAX_OBJECTS];
+};
+
+struct ForceArgs {
+    float gravity, buoyancy, dt, max_v2, gate;
+    int gate_on;
+};
+
+// ------------------------------------------------------------ launchers
+// Every launcher enqueues on `st` and returns hipGetLastError().
+
+hipError_t fill_f32(float* p, long long n, float value, hipStream_t st);
+hipError_t fill_u8(uint8_t* p, long long n, uint8_t value, hipStream_t st);
+
+// Contiguous (reference layout) <-> padded layout.
+hipError_t unpack_cells_f32(const Grid& g, const float* src, float* dst, hipStream_t st);
+hipError_t pack_cells_f32(const Grid& g, const float* src, float* dst, hipStream_t st);
+hipError_t unpack_faces_f32(const Grid& g, const float* src, float* dst, hipStream_t st);
+hipError_t pack_faces_f32(const Grid& g, const float* src, float* dst, hipStream_t st);
+hipError_t unpack_cells_solid(const Grid& g, const uint8_t* solid, uint8_t* base, hipStream_t st);
+hipError_t pack_cells_u8(const Grid& g, const uint8_t* src, uint8_t* dst, hipStream_t st);
+hipError_t floor_mask(const Grid& g, uint8_t* base, hipStream_t st);
+
+// Scene: solid mask from base + obstacles, then flags; sources.
+hipError_t build_mask(const Grid& g, const uint8_t* base, uint8_t* s, const SphereList& obst,
+                      int mode, hipStream_t st);
+hipError_t build_flags(const Grid& g, const uint8_t* s, uint8_t* flags, hipStream_t st);
+hipError_t fill_source(const Grid& g, float* rho0, float* rho1, float4 sph, hipStream_t st,
+                       int* launched);
+
+// Force: buoyancy + clamp on [now].
+hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
+                 const uint8_t* flags, const ForceArgs& a, hipStream_t st);
+
+// Projection.
+hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
+                                  const uint8_t* flags, int colour, double omega, hipStream_t st);
+hipError_t divergence(const Grid& g, const float* u, const float* v, const float* w,
+                      const uint8_t* flags, float* d, hipStream_t st);
+hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const float* d,
+                        const uint8_t* flags, float omega, int plain, int zero_in, hipStream_t st);
+hipError_t rbsor_half(const Grid& g, float* p, const float* d, const uint8_t* flags, int colour,
+                      float omega, hipStream_t st);
+hipError_t mask_pressure(const Grid& g, float* p, const uint8_t* flags, hipStream_t st);
+hipError_t gradient(const Grid& g, float* u, float* v, float* w, const float* p,
+                    const uint8_t* flags, hipStream_t st);
+
+// Advection.
+hipError_t advect_velocity(const Grid& g, const float* u0, const float* v0, const float* w0,
+                           float* u1, float* v1, float* w1, const uint8_t* flags, float dt,
+                           hipStream_t st);
+hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float* u,
+                          const float* v, const float* w, const uint8_t* flags, float dt,
+                          hipStream_t st);
+
+}  // namespace smk

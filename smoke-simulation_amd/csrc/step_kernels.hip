@@ -1,0 +1,643 @@
+// HIP kernels of one smoke time step, FP32, for gfx950.
+//
+// Each kernel evaluates every output value with the same operations, in the
+// same order, as the CPU oracle (oracle/smoke_oracle.cpp), which restates
+// project/smokeSimulation.cu.  This file is compiled with -ffp-contract=off so
+// no multiply-add is fused behind the source's back; FP32 division is the
+// correctly rounded one (HIP's default).  Results are therefore bit-identical
+// to the oracle's FP32 mode.
+//
+// Thread blocks are 64 x 4 x 1: one wave64 covers 64 consecutive x of one row,
+// so every field access of a wave is one contiguous 256-byte run (FP32) of a
+// 128-byte-aligned row.  blockIdx.z walks z planes.
+#include "smoke_internal.h"
+
+namespace smk {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+inline dim3 grid_for(int wx, int wy, int wz)
+{
+    return dim3((unsigned)((wx + BX - 1) / BX), (unsigned)((wy + BY - 1) / BY), (unsigned)wz);
+}
+
+// ------------------------------------------------------------ utilities
+
+__global__ void k_fill_f32(float* __restrict__ p, long long n, float value)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = value;
+}
+
+__global__ void k_fill_u8(uint8_t* __restrict__ p, long long n, uint8_t value)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = value;
+}
+
+// Contiguous index of the reference layout with row length rx and rows ry.
+__device__ __forceinline__ long long cidx(int x, int y, int z, int rx, int ry)
+{
+    return x + (long long)rx * (y + (long long)ry * z);
+}
+
+template <bool TO_PADDED, typename T>
+__global__ void k_repack(Grid g, const T* __restrict__ src, T* __restrict__ dst, int wx, int wy,
+                         int wz)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= wx || y >= wy || z >= wz) return;
+    int i = g.idx(x, y, z);
+    long long c = cidx(x, y, z, wx, wy);
+    if (TO_PADDED) dst[i] = src[c];
+    else dst[c] = src[i];
+}
+
+__global__ void k_unpack_solid(Grid g, const uint8_t* __restrict__ solid, uint8_t* __restrict__ base)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    base[g.idx(x, y, z)] = solid[cidx(x, y, z, g.nx, g.ny)] ? 0 : 1;
+}
+
+// Initial mask: fluid everywhere, solid floor y = 0 (smokeSimulation.cu:200-207).
+__global__ void k_floor_mask(Grid g, uint8_t* __restrict__ base)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    base[g.idx(x, y, z)] = (y == 0) ? 0 : 1;
+}
+
+// --------------------------------------------------------------- scene
+
+// Sphere membership as fillSmoke/fillObstacle test it
+// (smokeSimulation.cu:265, :304): integer cell index, float centre, squares
+// and sums in float, strict comparison with r*r.
+__device__ __forceinline__ bool inside(float4 s, int x, int y, int z)
+{
+    float dx = (float)x - s.x;
+    float dy = (float)y - s.y;
+    float dz = (float)z - s.z;
+    float dist = dx * dx + dy * dy + dz * dz;
+    return dist < s.w * s.w;
+}
+
+__device__ __forceinline__ bool is_interior(const Grid& g, int x, int y, int z)
+{
+    return x >= 1 && x <= g.nx - 2 && y >= 1 && y <= g.ny - 2 && z >= 1 && z <= g.nz - 2;
+}
+
+// fillObstacle (smokeSimulation.cu:289-313) over a base mask: interior cells
+// only; LAST_WINS keeps the reference's overwrite-per-obstacle result.
+__global__ void k_mask(Grid g, const uint8_t* __restrict__ base, uint8_t* __restrict__ s,
+                       SphereList obst, int mode)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    int i = g.idx(x, y, z);
+    uint8_t v = base[i];
+    if (obst.count > 0 && is_interior(g, x, y, z)) {
+        bool solid;
+        if (mode == SMOKE_OBSTACLES_LAST_WINS) {
+            solid = inside(obst.s[obst.count - 1], x, y, z);
+        } else {
+            solid = false;
+            for (int k = 0; k < obst.count; ++k) solid = solid || inside(obst.s[k], x, y, z);
+        }
+        if (solid) v = 0;
+    }
+    s[i] = v;
+}
+
+__global__ void k_flags(Grid g, const uint8_t* __restrict__ s, uint8_t* __restrict__ flags)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = s[i] ? F_SELF : 0;
+    if (x > 0 && s[i - 1]) fl |= F_XM;
+    if (x < g.nx - 1 && s[i + 1]) fl |= F_XP;
+    if (y > 0 && s[i - g.px]) fl |= F_YM;
+    if (y < g.ny - 1 && s[i + g.px]) fl |= F_YP;
+    if (z > 0 && s[i - g.plane]) fl |= F_ZM;
+    if (z < g.nz - 1 && s[i + g.plane]) fl |= F_ZP;
+    if (is_interior(g, x, y, z) && (fl & F_SELF) && (fl & F_NBRS)) fl |= F_ACTIVE;
+    flags[i] = fl;
+}
+
+// fillSmoke (smokeSimulation.cu:251-273) restricted to the bounding box of
+// one source: every interior cell inside it gets density 1.0 in both halves.
+__global__ void k_source(Grid g, float* __restrict__ r0, float* __restrict__ r1, float4 sph,
+                         int x0, int y0, int z0, int ex, int ey)
+{
+    int x = x0 + blockIdx.x * BX + threadIdx.x;
+    int y = y0 + blockIdx.y * BY + threadIdx.y;
+    int z = z0 + blockIdx.z;
+    if (x >= x0 + ex || y >= y0 + ey) return;
+    if (inside(sph, x, y, z)) {
+        int i = g.idx(x, y, z);
+        r0[i] = 1.0f;
+        r1[i] = 1.0f;
+    }
+}
+
+// --------------------------------------------------------------- force
+
+// integrate (smokeSimulation.cu:315-329) and velocityConfinement (:331-352)
+// fused per staggered index: the buoyancy range is x in [0,nx-1],
+// y in [1,ny-1], z in [0,nz-1]; the clamp range x,y,z in [1,n-1] lies inside
+// it and sees the post-buoyancy v of the same index.
+__global__ void k_force(Grid g, float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
+                        const float* __restrict__ rho, const uint8_t* __restrict__ flags, ForceArgs a)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    float vv = v[i];
+    bool write_v = false;
+    if ((fl & F_SELF) && (fl & F_YM)) {
+        float r = rho[i];
+        if (!a.gate_on || r > a.gate) {
+            float b = a.buoyancy * (r - 0.0f);
+            vv = vv + (r * a.gravity * a.dt + b * a.dt);
+            write_v = true;
+        }
+    }
+    if (x >= 1 && z >= 1) {
+        float uu = u[i];
+        float ww = w[i];
+        float L = uu * uu + vv * vv + ww * ww;
+        if (L * a.dt > a.max_v2) {
+            u[i] = uu * (a.max_v2 / (L * a.dt));
+            vv = vv * (a.max_v2 / (L * a.dt));
+            w[i] = ww * (a.max_v2 / (L * a.dt));
+            write_v = true;
+        }
+    }
+    if (write_v) v[i] = vv;
+}
+
+// ----------------------------------------------------------- projection
+
+// Colour-o cells of row (y,z): x = 2t + 1 + ((y+z+o+1) & 1) enumerates the
+// interior x with (x+y+z+o) even, as smokeSimulation.cu:362 does.
+__device__ __forceinline__ int colour_x(int t, int y, int z, int o)
+{
+    return 2 * t + 1 + ((y + z + o + 1) & 1);
+}
+
+// divergence (smokeSimulation.cu:356-394): one red-black SOR half-sweep in
+// place on the face velocities.  Same-colour cells share no face, so the
+// in-place update is race-free.
+__global__ void k_project_ref_half(Grid g, float* __restrict__ u, float* __restrict__ v,
+                                   float* __restrict__ w, const uint8_t* __restrict__ flags,
+                                   int colour, double omega)
+{
+    int t = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (y > g.ny - 2 || z > g.nz - 2) return;
+    int x = colour_x(t, y, z, colour);
+    if (x > g.nx - 2) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    if (!(fl & F_ACTIVE)) return;  // solid (:365) or no fluid neighbour (:377)
+    int sx0 = (fl >> 1) & 1, sx1 = (fl >> 2) & 1;
+    int sy0 = (fl >> 3) & 1, sy1 = (fl >> 4) & 1;
+    int sz0 = (fl >> 5) & 1, sz1 = (fl >> 6) & 1;
+    int acc = sx0 + sx1 + sy0 + sy1 + sz0 + sz1;
+    float dv = -u[i] + u[i + 1] + -v[i] + v[i + g.px] + -w[i] + w[i + g.plane];
+    float p = (float)((double)(-dv / (float)acc) * omega);
+    u[i] -= p * (float)sx0;
+    u[i + 1] += p * (float)sx1;
+    v[i] -= p * (float)sy0;
+    v[i + g.px] += p * (float)sy1;
+    w[i] -= p * (float)sz0;
+    w[i + g.plane] += p * (float)sz1;
+}
+
+// Right-hand side of the pressure form: d = div(u*) on active cells, in the
+// reference's operand order; 0 elsewhere.  Interior cells only.
+__global__ void k_divergence(Grid g, const float* __restrict__ u, const float* __restrict__ v,
+                             const float* __restrict__ w, const uint8_t* __restrict__ flags,
+                             float* __restrict__ d)
+{
+    int x = blockIdx.x * BX + threadIdx.x + 1;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2) return;
+    int i = g.idx(x, y, z);
+    float dv = 0.0f;
+    if (flags[i] & F_ACTIVE)
+        dv = -u[i] + u[i + 1] + -v[i] + v[i + g.px] + -w[i] + w[i + g.plane];
+    d[i] = dv;
+}
+
+__device__ __forceinline__ float nbr_sum(const float* __restrict__ p, int i, uint8_t fl, int px,
+                                         int plane)
+{
+    float sum = 0.0f;
+    if (fl & F_XM) sum += p[i - 1];
+    if (fl & F_XP) sum += p[i + 1];
+    if (fl & F_YM) sum += p[i - px];
+    if (fl & F_YP) sum += p[i + px];
+    if (fl & F_ZM) sum += p[i - plane];
+    if (fl & F_ZP) sum += p[i + plane];
+    return sum;
+}
+
+// One (damped) Jacobi sweep of acc_i P_i - sum_n s_n P_n = -d_i over the
+// interior; inactive cells are written 0.  zero_in: the input is all zeros
+// (cold start), so it is not read.
+__global__ void k_jacobi(Grid g, const float* __restrict__ pin, float* __restrict__ pout,
+                         const float* __restrict__ d, const uint8_t* __restrict__ flags,
+                         float omega, int plain, int zero_in)
+{
+    int x = blockIdx.x * BX + threadIdx.x + 1;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    float out = 0.0f;
+    if (fl & F_ACTIVE) {
+        float sum = zero_in ? 0.0f : nbr_sum(pin, i, fl, g.px, g.plane);
+        float G = (sum - d[i]) / (float)__popc(fl & F_NBRS);
+        if (plain) {
+            out = G;
+        } else {
+            float pi = zero_in ? 0.0f : pin[i];
+            out = pi + omega * (G - pi);
+        }
+    }
+    pout[i] = out;
+}
+
+// One red-black SOR half-sweep of the pressure form, in place.
+__global__ void k_rbsor_half(Grid g, float* __restrict__ p, const float* __restrict__ d,
+                             const uint8_t* __restrict__ flags, int colour, float omega)
+{
+    int t = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (y > g.ny - 2 || z > g.nz - 2) return;
+    int x = colour_x(t, y, z, colour);
+    if (x > g.nx - 2) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    if (!(fl & F_ACTIVE)) return;
+    float G = (nbr_sum(p, i, fl, g.px, g.plane) - d[i]) / (float)__popc(fl & F_NBRS);
+    p[i] = p[i] + omega * (G - p[i]);
+}
+
+// Warm start: zero the pressure of cells that are not active.
+__global__ void k_mask_pressure(Grid g, float* __restrict__ p, const uint8_t* __restrict__ flags)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    int i = g.idx(x, y, z);
+    if (!(flags[i] & F_ACTIVE)) p[i] = 0.0f;
+}
+
+// u -= grad P on every face whose two cells are fluid.  Faces that touch no
+// active cell see P_b - P_a = 0 and are skipped (x,y,z >= 1 suffices).
+__global__ void k_gradient(Grid g, float* __restrict__ u, float* __restrict__ v,
+                           float* __restrict__ w, const float* __restrict__ p,
+                           const uint8_t* __restrict__ flags)
+{
+    int x = blockIdx.x * BX + threadIdx.x + 1;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (x > g.nx - 1 || y > g.ny - 1 || z > g.nz - 1) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    if (!(fl & F_SELF)) return;
+    float pc = p[i];
+    if (fl & F_XM) u[i] = u[i] - (pc - p[i - 1]);
+    if (fl & F_YM) v[i] = v[i] - (pc - p[i - g.px]);
+    if (fl & F_ZM) w[i] = w[i] - (pc - p[i - g.plane]);
+}
+
+// ------------------------------------------------------------ advection
+
+// avgU/avgV/avgW (smokeSimulation.cu:409-447): 8-point means in the
+// reference's summation order.
+__device__ __forceinline__ float avgU(const float* __restrict__ a, int i, int px, int pl)
+{
+    float t = a[i - pl] + a[i + 1 - pl] + a[i - px - pl] + a[i + 1 - px - pl] + a[i] + a[i + 1] +
+              a[i - px] + a[i + 1 - px];
+    return t / 8.0f;
+}
+__device__ __forceinline__ float avgV(const float* __restrict__ a, int i, int px, int pl)
+{
+    float t = a[i - pl] + a[i - 1 - pl] + a[i + px - pl] + a[i - 1 + px - pl] + a[i] + a[i - 1] +
+              a[i + px] + a[i - 1 + px];
+    return t / 8.0f;
+}
+__device__ __forceinline__ float avgW(const float* __restrict__ a, int i, int px, int pl)
+{
+    float t = a[i] + a[i - 1] + a[i - px] + a[i - 1 - px] + a[i - pl] + a[i - 1 - pl] +
+              a[i - px - pl] + a[i - 1 - px - pl];
+    return t / 8.0f;
+}
+
+// sampleSmoke (smokeSimulation.cu:451-484): clamp to [1, n-1] with the cell
+// dims, trilinear weights, fixed corner order.
+__device__ __forceinline__ float sample(const Grid& g, const float* __restrict__ a, float px_,
+                                        float py_, float pz_, float dx, float dy, float dz)
+{
+    float x = fmaxf(fminf(px_, (float)(g.nx - 1)), 1.0f);
+    float y = fmaxf(fminf(py_, (float)(g.ny - 1)), 1.0f);
+    float z = fmaxf(fminf(pz_, (float)(g.nz - 1)), 1.0f);
+    int x0 = (int)fminf(floorf(x - dx), (float)(g.nx - 1));
+    float xw1 = (x - dx) - (float)x0;
+    float xw0 = 1.0f - xw1;
+    int x1 = (int)fminf((float)(x0 + 1), (float)(g.nx - 1));
+    int y0 = (int)fminf(floorf(y - dy), (float)(g.ny - 1));
+    float yw1 = (y - dy) - (float)y0;
+    float yw0 = 1.0f - yw1;
+    int y1 = (int)fminf((float)(y0 + 1), (float)(g.ny - 1));
+    int z0 = (int)fminf(floorf(z - dz), (float)(g.nz - 1));
+    float zw1 = (z - dz) - (float)z0;
+    float zw0 = 1.0f - zw1;
+    int z1 = (int)fminf((float)(z0 + 1), (float)(g.nz - 1));
+    return xw0 * yw0 * zw0 * a[g.idx(x0, y0, z0)] + xw1 * yw0 * zw0 * a[g.idx(x1, y0, z0)] +
+           xw0 * yw1 * zw0 * a[g.idx(x0, y1, z0)] + xw1 * yw1 * zw0 * a[g.idx(x1, y1, z0)] +
+           xw0 * yw0 * zw1 * a[g.idx(x0, y0, z1)] + xw1 * yw0 * zw1 * a[g.idx(x1, y0, z1)] +
+           xw0 * yw1 * zw1 * a[g.idx(x0, y1, z1)] + xw1 * yw1 * zw1 * a[g.idx(x1, y1, z1)];
+}
+
+// velocityAdvectionU/V/W (smokeSimulation.cu:527-615) fused per staggered
+// index over x,y,z in [1,n-1]: each component keeps its own range and
+// fluid-fluid condition; read [now], write [past].
+__global__ void k_advect_velocity(Grid g, const float* __restrict__ u0, const float* __restrict__ v0,
+                                  const float* __restrict__ w0, float* __restrict__ u1,
+                                  float* __restrict__ v1, float* __restrict__ w1,
+                                  const uint8_t* __restrict__ flags, float dt)
+{
+    int x = blockIdx.x * BX + threadIdx.x + 1;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (x > g.nx - 1 || y > g.ny - 1 || z > g.nz - 1) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    if (!(fl & F_SELF)) return;
+    const int px = g.px, pl = g.plane;
+    if (y < g.ny - 1 && z < g.nz - 1 && (fl & F_XM)) {
+        float xt = (float)x, yt = (float)y + 0.5f, zt = (float)z + 0.5f;
+        float uu = u0[i];
+        float vv = avgV(v0, i, px, pl);
+        float ww = avgW(w0, i, px, pl);
+        xt = xt - dt * uu;
+        yt = yt - dt * vv;
+        zt = zt - dt * ww;
+        u1[i] = sample(g, u0, xt, yt, zt, 0.0f, 0.5f, 0.5f);
+    }
+    if (x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM)) {
+        float xt = (float)x + 0.5f, yt = (float)y, zt = (float)z + 0.5f;
+        float uu = avgU(u0, i, px, pl);
+        float vv = v0[i];
+        float ww = avgW(w0, i, px, pl);
+        xt = xt - dt * uu;
+        yt = yt - dt * vv;
+        zt = zt - dt * ww;
+        v1[i] = sample(g, v0, xt, yt, zt, 0.5f, 0.0f, 0.5f);
+    }
+    if (x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM)) {
+        float xt = (float)x + 0.5f, yt = (float)y + 0.5f, zt = (float)z;
+        float uu = avgU(u0, i, px, pl);
+        float vv = avgV(v0, i, px, pl);
+        float ww = w0[i];
+        xt = xt - dt * uu;
+        yt = yt - dt * vv;
+        zt = zt - dt * ww;
+        w1[i] = sample(g, w0, xt, yt, zt, 0.5f, 0.5f, 0.0f);
+    }
+}
+
+// advectSmoke (smokeSimulation.cu:617-638): cell velocity from the
+// post-advection faces, back-trace in double rounded to float.
+__global__ void k_advect_density(Grid g, const float* __restrict__ r0, float* __restrict__ r1,
+                                 const float* __restrict__ u, const float* __restrict__ v,
+                                 const float* __restrict__ w, const uint8_t* __restrict__ flags,
+                                 float dt)
+{
+    int x = blockIdx.x * BX + threadIdx.x + 1;
+    int y = blockIdx.y * BY + threadIdx.y + 1;
+    int z = blockIdx.z + 1;
+    if (x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2) return;
+    int i = g.idx(x, y, z);
+    if (!(flags[i] & F_SELF)) return;
+    float ut = (u[i] + u[i + 1]) / 2.0f;
+    float vt = (v[i] + v[i + g.px]) / 2.0f;
+    float wt = (w[i] + w[i + g.plane]) / 2.0f;
+    float xt = (float)((double)x + 0.5 - (double)(ut * dt));
+    float yt = (float)((double)y + 0.5 - (double)(vt * dt));
+    float zt = (float)((double)z + 0.5 - (double)(wt * dt));
+    r1[i] = sample(g, r0, xt, yt, zt, 0.5f, 0.5f, 0.5f);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+hipError_t fill_f32(float* p, long long n, float value, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t fill_u8(uint8_t* p, long long n, uint8_t value, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)blocks), dim3(256), 0, st, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t unpack_cells_f32(const Grid& g, const float* src, float* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_repack<true, float>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,
+                       g, src, dst, g.nx, g.ny, g.nz);
+    return hipGetLastError();
+}
+
+hipError_t pack_cells_f32(const Grid& g, const float* src, float* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_repack<false, float>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,
+                       g, src, dst, g.nx, g.ny, g.nz);
+    return hipGetLastError();
+}
+
+hipError_t unpack_faces_f32(const Grid& g, const float* src, float* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_repack<true, float>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),
+                       dim3(BX, BY), 0, st, g, src, dst, g.nx + 1, g.ny + 1, g.nz + 1);
+    return hipGetLastError();
+}
+
+hipError_t pack_faces_f32(const Grid& g, const float* src, float* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_repack<false, float>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),
+                       dim3(BX, BY), 0, st, g, src, dst, g.nx + 1, g.ny + 1, g.nz + 1);
+    return hipGetLastError();
+}
+
+hipError_t unpack_cells_solid(const Grid& g, const uint8_t* solid, uint8_t* base, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_unpack_solid, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, solid,
+                       base);
+    return hipGetLastError();
+}
+
+hipError_t pack_cells_u8(const Grid& g, const uint8_t* src, uint8_t* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_repack<false, uint8_t>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,
+                       g, src, dst, g.nx, g.ny, g.nz);
+    return hipGetLastError();
+}
+
+hipError_t floor_mask(const Grid& g, uint8_t* base, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_floor_mask, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, base);
+    return hipGetLastError();
+}
+
+hipError_t build_mask(const Grid& g, const uint8_t* base, uint8_t* s, const SphereList& obst,
+                      int mode, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mask, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, base, s, obst,
+                       mode);
+    return hipGetLastError();
+}
+
+hipError_t build_flags(const Grid& g, const uint8_t* s, uint8_t* flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flags, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, s, flags);
+    return hipGetLastError();
+}
+
+hipError_t fill_source(const Grid& g, float* rho0, float* rho1, float4 sph, hipStream_t st,
+                       int* launched)
+{
+    *launched = 0;
+    // Conservative bounding box: any cell with d^2 < r^2 lies within
+    // ceil(|r|) + 1 of the centre on each axis; clipped to the interior.
+    double r = fabs((double)sph.w);
+    double c[3] = {(double)sph.x, (double)sph.y, (double)sph.z};
+    int n[3] = {g.nx, g.ny, g.nz};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        if (!(c[k] == c[k]) || !(r == r)) return hipSuccess;  // NaN: nothing inside
+        double l = std::floor(c[k] - r) - 1.0, h = std::ceil(c[k] + r) + 1.0;
+        lo[k] = l < 1.0 ? 1 : (l > n[k] - 2 ? n[k] - 1 : (int)l);
+        hi[k] = h > n[k] - 2 ? n[k] - 2 : (h < 1.0 ? 0 : (int)h);
+        if (lo[k] > hi[k]) return hipSuccess;
+    }
+    int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    hipLaunchKernelGGL(k_source, grid_for(ex, ey, ez), dim3(BX, BY), 0, st, g, rho0, rho1, sph,
+                       lo[0], lo[1], lo[2], ex, ey);
+    *launched = 1;
+    return hipGetLastError();
+}
+
+hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
+                 const uint8_t* flags, const ForceArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_force, grid_for(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
+                       rho, flags, a);
+    return hipGetLastError();
+}
+
+hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
+                                  const uint8_t* flags, int colour, double omega, hipStream_t st)
+{
+    int tx = (g.nx - 2 + 1) / 2;
+    hipLaunchKernelGGL(k_project_ref_half, grid_for(tx, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
+                       u, v, w, flags, colour, omega);
+    return hipGetLastError();
+}
+
+hipError_t divergence(const Grid& g, const float* u, const float* v, const float* w,
+                      const uint8_t* flags, float* d, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_divergence, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
+                       u, v, w, flags, d);
+    return hipGetLastError();
+}
+
+hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const float* d,
+                        const uint8_t* flags, float omega, int plain, int zero_in, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_jacobi, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
+                       pin, pout, d, flags, omega, plain, zero_in);
+    return hipGetLastError();
+}
+
+hipError_t rbsor_half(const Grid& g, float* p, const float* d, const uint8_t* flags, int colour,
+                      float omega, hipStream_t st)
+{
+    int tx = (g.nx - 2 + 1) / 2;
+    hipLaunchKernelGGL(k_rbsor_half, grid_for(tx, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g, p, d,
+                       flags, colour, omega);
+    return hipGetLastError();
+}
+
+hipError_t mask_pressure(const Grid& g, float* p, const uint8_t* flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mask_pressure, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, p,
+                       flags);
+    return hipGetLastError();
+}
+
+hipError_t gradient(const Grid& g, float* u, float* v, float* w, const float* p,
+                    const uint8_t* flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gradient, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st, g,
+                       u, v, w, p, flags);
+    return hipGetLastError();
+}
+
+hipError_t advect_velocity(const Grid& g, const float* u0, const float* v0, const float* w0,
+                           float* u1, float* v1, float* w1, const uint8_t* flags, float dt,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advect_velocity, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0,
+                       st, g, u0, v0, w0, u1, v1, w1, flags, dt);
+    return hipGetLastError();
+}
+
+hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float* u,
+                          const float* v, const float* w, const uint8_t* flags, float dt,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advect_density, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
+                       st, g, r0, r1, u, v, w, flags, dt);
+    return hipGetLastError();
+}
+
+}  // namespace smk

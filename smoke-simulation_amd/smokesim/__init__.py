@@ -1,0 +1,339 @@
+"""Python binding of libsmokesim.so, the MI355X smoke time step.
+
+A thin ctypes layer over the C ABI declared in ``include/smoke.h``; every
+method maps to one entry point.  Arrays are numpy arrays in the reference's
+layouts (project/smokeSimulation.cu:146-147, :267): density, masks and
+pressure shaped (nz, ny, nx); velocity components shaped (nz+1, ny+1, nx+1);
+x fastest.
+
+The library is required: importing this module on a machine where
+``libsmokesim.so`` is missing raises, and nothing here falls back to CPU code.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes  # noqa: F401  (re-exported helper module)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsmokesim.so")
+
+SMOKE_OK = 0
+PROJ_REFERENCE, PROJ_RBSOR, PROJ_JACOBI = 0, 1, 2
+OBSTACLES_LAST_WINS, OBSTACLES_UNION = 0, 1
+STAGE_SCENE, STAGE_FORCE, STAGE_PROJECT, STAGE_ADVECT_VEL, STAGE_ADVECT_DENS = range(5)
+STAGE_COUNT = 5
+STAGE_NAMES = ("scene", "force", "project", "advect_velocity", "advect_density")
+MAX_OBJECTS = 64
+
+F_SELF, F_XM, F_XP, F_YM, F_YP, F_ZM, F_ZP, F_ACTIVE = 1, 2, 4, 8, 16, 32, 64, 128
+
+
+class SmokeError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"smoke status {status}: {msg}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("gravity", ctypes.c_float),
+        ("buoyancy", ctypes.c_float),
+        ("density_gate", ctypes.c_float),
+        ("max_velocity", ctypes.c_float),
+        ("sor_omega", ctypes.c_double),
+        ("jacobi_omega", ctypes.c_double),
+        ("projection", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("obstacle_mode", ctypes.c_int),
+        ("warm_start", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def _declare(lib):
+    vp = ctypes.c_void_p
+    st = ctypes.c_int
+    i, u, f = ctypes.c_int, ctypes.c_uint, ctypes.c_float
+    fp = ctypes.POINTER(ctypes.c_float)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    dp = ctypes.POINTER(ctypes.c_double)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    ip = ctypes.POINTER(ctypes.c_int)
+    up = ctypes.POINTER(ctypes.c_uint)
+    pp = ctypes.POINTER(Params)
+    sig = {
+        "smoke_last_error": (ctypes.c_char_p, []),
+        "smoke_version": (ctypes.c_char_p, []),
+        "smoke_device_info": (st, [ctypes.c_char_p, ctypes.c_size_t]),
+        "smoke_params_default": (st, [pp]),
+        "smoke_create": (st, [u, u, u, fp, ctypes.POINTER(vp)]),
+        "smoke_destroy": (None, [vp]),
+        "smoke_get_dims": (st, [vp, up, up, up]),
+        "smoke_footprint_bytes": (ctypes.c_size_t, [u, u, u]),
+        "smoke_set_params": (st, [vp, pp]),
+        "smoke_get_params": (st, [vp, pp]),
+        "smoke_add_obstacle": (st, [vp, f, f, f, f, f, f, f, ip]),
+        "smoke_add_source": (st, [vp, f, f, f, f, ip]),
+        "smoke_move_object": (st, [vp, i, f, f, f]),
+        "smoke_set_solids": (st, [vp, u8p]),
+        "smoke_set_density": (st, [vp, i, fp]),
+        "smoke_get_density": (st, [vp, i, fp]),
+        "smoke_set_velocity": (st, [vp, i, i, fp]),
+        "smoke_get_velocity": (st, [vp, i, i, fp]),
+        "smoke_get_mask": (st, [vp, u8p]),
+        "smoke_get_flags": (st, [vp, u8p]),
+        "smoke_get_pressure": (st, [vp, fp]),
+        "smoke_index_now": (i, [vp]),
+        "smoke_step": (st, [vp, f, fp]),
+        "smoke_step_async": (st, [vp, f]),
+        "smoke_run_stage": (st, [vp, i, f]),
+        "smoke_read_density": (st, [vp, fp]),
+        "smoke_synchronize": (st, [vp]),
+        "smoke_timing_enable": (st, [vp, i]),
+        "smoke_timing_read": (st, [vp, dp, llp, llp]),
+        "smoke_timing_reset": (st, [vp]),
+        "smoke_timing_step_ms": (st, [vp, dp, i, ip]),
+        "smoke_timing_sweeps": (st, [vp, dp, llp]),
+        "smoke_legacy_device_info": (st, []),
+        "smoke_legacy_initialize": (st, [fp, u, u, u]),
+        "smoke_legacy_delete": (st, []),
+        "smoke_legacy_simulate": (st, [fp, f]),
+        "smoke_legacy_add_obstacle": (i, [f, f, f, f, f, f, f]),
+        "smoke_legacy_add_source": (i, [f, f, f, f]),
+        "smoke_legacy_update_object_pos": (st, [i, f, f, f]),
+        "smoke_legacy_buoyancy_ptr": (fp, []),
+        "smoke_legacy_gravity_ptr": (fp, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def load():
+    """Load libsmokesim.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libsmokesim.so not found at {LIB_PATH}; build it with "
+                "`make -C smoke-simulation_amd` (or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        _declare(lib)
+        _lib = lib
+    return _lib
+
+
+def _check(status):
+    if status != SMOKE_OK:
+        raise SmokeError(status, load().smoke_last_error().decode())
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u8ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(512)
+    _check(load().smoke_device_info(buf, len(buf)))
+    return buf.value.decode()
+
+
+def default_params():
+    p = Params()
+    _check(load().smoke_params_default(ctypes.byref(p)))
+    return p.as_dict()
+
+
+def footprint_bytes(nx, ny, nz):
+    return int(load().smoke_footprint_bytes(nx, ny, nz))
+
+
+class Simulation:
+    """One simulation on the current device (smoke_create ... smoke_destroy)."""
+
+    def __init__(self, nx, ny, nz, density=None, params=None):
+        self.lib = load()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        h = ctypes.c_void_p()
+        dptr = None
+        if density is not None:
+            d = np.ascontiguousarray(density, dtype=np.float32).reshape(self.cell_shape)
+            dptr = _fptr(d)
+        _check(self.lib.smoke_create(self.nx, self.ny, self.nz, dptr, ctypes.byref(h)))
+        self.h = h
+        if params:
+            self.set_params(**params)
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.smoke_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def cell_shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def face_shape(self):
+        return (self.nz + 1, self.ny + 1, self.nx + 1)
+
+    @property
+    def cells(self):
+        return self.nx * self.ny * self.nz
+
+    # params
+    def get_params(self):
+        p = Params()
+        _check(self.lib.smoke_get_params(self.h, ctypes.byref(p)))
+        return p.as_dict()
+
+    def set_params(self, **kw):
+        cur = self.get_params()
+        for k in kw:
+            if k not in cur:
+                raise KeyError(k)
+        cur.update(kw)
+        p = Params(**cur)
+        _check(self.lib.smoke_set_params(self.h, ctypes.byref(p)))
+
+    # scene
+    def add_obstacle(self, x, y, z, r, vel=(0.0, 0.0, 0.0)):
+        oid = ctypes.c_int()
+        _check(self.lib.smoke_add_obstacle(self.h, x, y, z, vel[0], vel[1], vel[2], r,
+                                           ctypes.byref(oid)))
+        return oid.value
+
+    def add_source(self, x, y, z, r):
+        oid = ctypes.c_int()
+        _check(self.lib.smoke_add_source(self.h, x, y, z, r, ctypes.byref(oid)))
+        return oid.value
+
+    def move_object(self, oid, x, y, z):
+        _check(self.lib.smoke_move_object(self.h, int(oid), x, y, z))
+
+    def set_solids(self, solid):
+        a = np.ascontiguousarray(solid, dtype=np.uint8).reshape(self.cell_shape)
+        _check(self.lib.smoke_set_solids(self.h, _u8ptr(a)))
+
+    # fields
+    def set_density(self, half, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(self.cell_shape)
+        _check(self.lib.smoke_set_density(self.h, int(half), _fptr(a)))
+
+    def get_density(self, half):
+        a = np.empty(self.cell_shape, dtype=np.float32)
+        _check(self.lib.smoke_get_density(self.h, int(half), _fptr(a)))
+        return a
+
+    def set_velocity(self, comp, half, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(self.face_shape)
+        _check(self.lib.smoke_set_velocity(self.h, int(comp), int(half), _fptr(a)))
+
+    def get_velocity(self, comp, half):
+        a = np.empty(self.face_shape, dtype=np.float32)
+        _check(self.lib.smoke_get_velocity(self.h, int(comp), int(half), _fptr(a)))
+        return a
+
+    def get_mask(self):
+        a = np.empty(self.cell_shape, dtype=np.uint8)
+        _check(self.lib.smoke_get_mask(self.h, _u8ptr(a)))
+        return a
+
+    def get_flags(self):
+        a = np.empty(self.cell_shape, dtype=np.uint8)
+        _check(self.lib.smoke_get_flags(self.h, _u8ptr(a)))
+        return a
+
+    def get_pressure(self):
+        a = np.empty(self.cell_shape, dtype=np.float32)
+        _check(self.lib.smoke_get_pressure(self.h, _fptr(a)))
+        return a
+
+    @property
+    def index_now(self):
+        return self.lib.smoke_index_now(self.h)
+
+    # stepping
+    def step(self, dt, readback=True):
+        if readback:
+            out = np.empty(self.cell_shape, dtype=np.float32)
+            _check(self.lib.smoke_step(self.h, float(dt), _fptr(out)))
+            return out
+        _check(self.lib.smoke_step(self.h, float(dt), None))
+        return None
+
+    def step_async(self, dt):
+        _check(self.lib.smoke_step_async(self.h, float(dt)))
+
+    def run_stage(self, stage, dt):
+        _check(self.lib.smoke_run_stage(self.h, int(stage), float(dt)))
+
+    def read_density(self):
+        out = np.empty(self.cell_shape, dtype=np.float32)
+        _check(self.lib.smoke_read_density(self.h, _fptr(out)))
+        return out
+
+    def synchronize(self):
+        _check(self.lib.smoke_synchronize(self.h))
+
+    # timing
+    def timing_enable(self, on=True):
+        _check(self.lib.smoke_timing_enable(self.h, 1 if on else 0))
+
+    def timing_reset(self):
+        _check(self.lib.smoke_timing_reset(self.h))
+
+    def timing_read(self):
+        ms = (ctypes.c_double * STAGE_COUNT)()
+        launches = (ctypes.c_longlong * STAGE_COUNT)()
+        steps = ctypes.c_longlong()
+        _check(self.lib.smoke_timing_read(self.h, ms, launches, ctypes.byref(steps)))
+        return {
+            "steps": steps.value,
+            "stage_ms": {STAGE_NAMES[k]: ms[k] for k in range(STAGE_COUNT)},
+            "stage_launches": {STAGE_NAMES[k]: launches[k] for k in range(STAGE_COUNT)},
+        }
+
+    def timing_sweeps(self):
+        """(total ms, launches) of the projection sweep loops of timed steps."""
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _check(self.lib.smoke_timing_sweeps(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def timing_step_ms(self, cap=1 << 16):
+        buf = (ctypes.c_double * cap)()
+        n = ctypes.c_int()
+        _check(self.lib.smoke_timing_step_ms(self.h, buf, cap, ctypes.byref(n)))
+        return np.array(buf[: n.value], dtype=np.float64)
+
+    def state(self):
+        """Every field of both halves, plus mask and flags (blocking)."""
+        out = {"mask": self.get_mask(), "flags": self.get_flags(), "now": self.index_now}
+        for h in (0, 1):
+            out[f"rho{h}"] = self.get_density(h)
+            for k, name in enumerate("uvw"):
+                out[f"{name}{h}"] = self.get_velocity(k, h)
+        return out
