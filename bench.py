@@ -167,8 +167,10 @@ def main():
     if sweep_launches > 0:
         avg_ms = sweep_ms / sweep_launches
         if p["projection"] == 2:
-            alg = BYTES_SWEEP * interior
-            kernel = "k_jacobi"
+            # fused launches: sweeps per launch = all timed sweeps / launches
+            per_launch = p["iterations"] * timing["steps"] / sweep_launches
+            alg = BYTES_SWEEP * interior * per_launch
+            kernel = "k_jacobi_fused"
         elif p["projection"] == 1:
             alg = BYTES_SWEEP * interior
             kernel = "k_rbsor_half"
@@ -181,6 +183,7 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2),
             "alg_bytes_per_launch": int(alg), "launches_timed": sweep_launches,
+            "jacobi_plan": sim.get_jacobi_plan() if p["projection"] == 2 else None,
         }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if roofline and os.path.exists(traffic_file):

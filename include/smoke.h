@@ -121,6 +121,20 @@ size_t smoke_footprint_bytes(unsigned nx, unsigned ny, unsigned nz);
 smoke_status smoke_set_params(smoke_sim* sim, const smoke_params* p);
 smoke_status smoke_get_params(const smoke_sim* sim, smoke_params* out);
 
+/* Launch plan of the fused Jacobi sweeps (SMOKE_PROJ_JACOBI): how many sweeps
+ * one launch fuses (1..6), the thread-block tile and the z-chunk length.
+ * Every plan gives bit-identical results; only the speed differs.  No
+ * reference counterpart (the reference launches one kernel per half-sweep,
+ * smokeSimulation.cu:797-801). */
+typedef enum smoke_tile { SMOKE_TILE_32x32 = 0, SMOKE_TILE_64x16 = 1 } smoke_tile;
+typedef struct smoke_jacobi_plan {
+    int sweeps_per_launch; /* 1..6 (default 4) */
+    int tile;              /* smoke_tile (default 32x32) */
+    int zc;                /* z planes per block, >= 1 (default 128) */
+} smoke_jacobi_plan;
+smoke_status smoke_set_jacobi_plan(smoke_sim* sim, const smoke_jacobi_plan* plan);
+smoke_status smoke_get_jacobi_plan(const smoke_sim* sim, smoke_jacobi_plan* out);
+
 /* ---------------------------------------------------------------- scene */
 
 /* Replace addObstacle / addSmokeSource (smokeSimulation.cu:88-104).  Ids are

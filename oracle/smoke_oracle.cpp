@@ -46,7 +46,8 @@
 //                                        pressure form, convergence to a dense
 //                                        direct solve
 //   pressure-form Jacobi / RB-SOR ...... dense direct solve at convergence,
-//                                        agreement with each other
+//                                        agreement with each other, one hand
+//                                        sweep from P = 0
 // No reference-held golden vector exists for this path (the reference ships
 // no tests or fixtures), so none of these is pinned against reference output.
 
@@ -382,7 +383,8 @@ struct Sim : SimBase {
     }
 
     // Neighbour sum of the pressure form: masked by the neighbour bits, in the
-    // fixed order x-, x+, y-, y+, z-, z+.
+    // fixed order x-, x+, y-, y+, z-, z+.  (P is +0 on every cell that is not
+    // active, so summing all six neighbours unmasked gives the same bits.)
     Real nbr_sum(const std::vector<Real>& A, long ci, uint8_t fl) const
     {
         Real sum = Real(0);
@@ -395,10 +397,15 @@ struct Sim : SimBase {
         return sum;
     }
 
+    // Diagonal inverse 1/acc, rounded to Real (the pressure forms multiply by
+    // it instead of dividing).
+    static Real inv_acc(int acc) { return Real(1) / (Real)acc; }
+
     // Pressure form of the same projection (derivation: each face of the
     // reference's in-place update equals u* - s_a s_b (P_b - P_a) with P the
     // accumulated per-cell corrections, so the colour sweep is SOR on
-    // acc_i P_i - sum_n s_n P_n = -div_i(u*)).
+    // acc_i P_i - sum_n s_n P_n = -div_i(u*)).  The update of a cell is
+    // G = (sum_n s_n P_n - d_i) * (1/acc_i).
     void project_pressure()
     {
         // Right-hand side: d = div(u*) on active cells.
@@ -427,7 +434,7 @@ struct Sim : SimBase {
                             long ci = c(x, y, z);
                             uint8_t fl = flags[ci];
                             if (!(fl & F_ACTIVE)) { B[ci] = Real(0); continue; }
-                            Real G = (nbr_sum(A, ci, fl) - d[ci]) / (Real)acc_of(fl);
+                            Real G = (nbr_sum(A, ci, fl) - d[ci]) * inv_acc(acc_of(fl));
                             B[ci] = plain ? G : A[ci] + om * (G - A[ci]);
                         }
                 P.swap(Q);
@@ -444,7 +451,7 @@ struct Sim : SimBase {
                                 long ci = c(x, y, z);
                                 uint8_t fl = flags[ci];
                                 if (!(fl & F_ACTIVE)) continue;
-                                Real G = (nbr_sum(P, ci, fl) - d[ci]) / (Real)acc_of(fl);
+                                Real G = (nbr_sum(P, ci, fl) - d[ci]) * inv_acc(acc_of(fl));
                                 P[ci] = P[ci] + om * (G - P[ci]);
                             }
                 }

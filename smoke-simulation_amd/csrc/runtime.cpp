@@ -125,6 +125,7 @@ struct smoke_sim {
     int now = 1;   // indexNow (smokeSimulation.cu:707)
     int pcur = 0;  // P buffer holding the latest pressure
     bool mask_dirty = true;
+    smk::JacobiPlan plan;  // launch plan of the fused Jacobi sweeps
 
     // Timing: events at every stage boundary of a step, plus two around the
     // projection's sweep launches (the step's dominant kernel).
@@ -252,16 +253,22 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
             *launches += 1;
         }
         if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
-        for (int it = 0; it < p.iterations; ++it) {
-            int out = 1 - s->pcur;
-            e = smk::jacobi_sweep(g, s->P[s->pcur], s->P[out], s->d, s->flags, (float)p.jacobi_omega,
-                                  plain ? 1 : 0, zero_in ? 1 : 0, s->stream);
+        // K sweeps as launches of plan.sweeps_per_launch fused sweeps each,
+        // the remainder in one shorter launch.
+        long long sweep_launches = 0;
+        for (int left = p.iterations; left > 0;) {
+            const int t = std::min(left, s->plan.sweeps_per_launch);
+            const int out = 1 - s->pcur;
+            e = smk::jacobi_fused(g, s->P[s->pcur], s->P[out], s->d, s->flags, s->plan, t,
+                                  (float)p.jacobi_omega, plain ? 1 : 0, zero_in ? 1 : 0, s->stream);
             if (e != hipSuccess) return e;
             *launches += 1;
+            sweep_launches += 1;
             s->pcur = out;
             zero_in = false;
+            left -= t;
         }
-        if ((e = sweep_mark(s, 1, p.iterations)) != hipSuccess) return e;
+        if ((e = sweep_mark(s, 1, sweep_launches)) != hipSuccess) return e;
     } else {
         if (p.warm_start) e = smk::mask_pressure(g, s->P[s->pcur], s->flags, s->stream);
         else e = smk::fill_f32(s->P[s->pcur], g.elems, 0.0f, s->stream);
@@ -534,6 +541,31 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
     std::lock_guard<std::mutex> lk(s->mu);
     if (p->obstacle_mode != s->prm.obstacle_mode) s->mask_dirty = true;
     s->prm = *p;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_set_jacobi_plan(smoke_sim* s, const smoke_jacobi_plan* plan)
+{
+    if (!s || !plan) return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: NULL");
+    if (plan->sweeps_per_launch < 1 || plan->sweeps_per_launch > 6)
+        return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: sweeps_per_launch %d not in 1..6",
+                    plan->sweeps_per_launch);
+    if (plan->tile != SMOKE_TILE_32x32 && plan->tile != SMOKE_TILE_64x16)
+        return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: unknown tile %d", plan->tile);
+    if (plan->zc < 1) return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: zc %d < 1", plan->zc);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->plan.sweeps_per_launch = plan->sweeps_per_launch;
+    s->plan.tile = plan->tile;
+    s->plan.zc = plan->zc;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_jacobi_plan(const smoke_sim* s, smoke_jacobi_plan* out)
+{
+    if (!s || !out) return fail(SMOKE_ERR_INVALID, "get_jacobi_plan: NULL");
+    out->sweeps_per_launch = s->plan.sweeps_per_launch;
+    out->tile = s->plan.tile;
+    out->zc = s->plan.zc;
     return SMOKE_OK;
 }
 

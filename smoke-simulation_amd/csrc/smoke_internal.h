@@ -52,6 +52,16 @@ struct ForceArgs {
     int gate_on;
 };
 
+// Launch plan of the fused Jacobi kernel: sweeps fused per launch (2..6),
+// thread-block tile and z-chunk length.  Every plan gives bit-identical
+// results; the choice only changes speed.
+struct JacobiPlan {
+    enum Tile { TILE_32x32 = 0, TILE_64x16 = 1 };
+    int sweeps_per_launch = 4;
+    int tile = TILE_32x32;
+    int zc = 128;
+};
+
 // ------------------------------------------------------------ launchers
 // Every launcher enqueues on `st` and returns hipGetLastError().
 
@@ -85,6 +95,10 @@ hipError_t divergence(const Grid& g, const float* u, const float* v, const float
                       const uint8_t* flags, float* d, hipStream_t st);
 hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const float* d,
                         const uint8_t* flags, float omega, int plain, int zero_in, hipStream_t st);
+// `sweeps` (1..6) Jacobi sweeps in one launch (1: the single-sweep kernel).
+hipError_t jacobi_fused(const Grid& g, const float* pin, float* pout, const float* d,
+                        const uint8_t* flags, const JacobiPlan& plan, int sweeps, float omega,
+                        int plain, int zero_in, hipStream_t st);
 hipError_t rbsor_half(const Grid& g, float* p, const float* d, const uint8_t* flags, int colour,
                       float omega, hipStream_t st);
 hipError_t mask_pressure(const Grid& g, float* p, const uint8_t* flags, hipStream_t st);

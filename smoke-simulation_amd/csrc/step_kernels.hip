@@ -264,6 +264,41 @@ __device__ __forceinline__ float nbr_sum(const float* __restrict__ p, int i, uin
     return sum;
 }
 
+// RN(1/acc) for acc = 1..6: compile-time constants (correctly rounded by the
+// compiler) picked with selects, so no memory is touched.
+__device__ __forceinline__ float inv_acc(int acc)
+{
+    float r = 1.0f;
+    r = (acc == 2) ? 0.5f : r;
+    r = (acc == 3) ? (1.0f / 3.0f) : r;
+    r = (acc == 4) ? 0.25f : r;
+    r = (acc == 5) ? 0.2f : r;
+    r = (acc == 6) ? (1.0f / 6.0f) : r;
+    return r;
+}
+
+// Diagonal inverse of a cell, 0 for cells the projection does not relax.
+__device__ __forceinline__ float cell_inv(uint8_t fl)
+{
+    return (fl & F_ACTIVE) ? inv_acc(__popc(fl & F_NBRS)) : 0.0f;
+}
+
+// The pressure is +0 on every cell that is not active (ring, solid and
+// isolated cells are written 0 and never relaxed), and a running sum that
+// starts at +0 is never -0, so adding all six neighbours unmasked gives the
+// same bits as adding only the fluid ones.
+__device__ __forceinline__ float sum6(float a0, float a1, float a2, float a3, float a4, float a5)
+{
+    float sum = 0.0f;
+    sum = sum + a0;
+    sum = sum + a1;
+    sum = sum + a2;
+    sum = sum + a3;
+    sum = sum + a4;
+    sum = sum + a5;
+    return sum;
+}
+
 // One (damped) Jacobi sweep of acc_i P_i - sum_n s_n P_n = -d_i over the
 // interior; inactive cells are written 0.  zero_in: the input is all zeros
 // (cold start), so it is not read.
@@ -276,19 +311,120 @@ __global__ void k_jacobi(Grid g, const float* __restrict__ pin, float* __restric
     int z = blockIdx.z + 1;
     if (x > g.nx - 2 || y > g.ny - 2 || z > g.nz - 2) return;
     int i = g.idx(x, y, z);
-    uint8_t fl = flags[i];
-    float out = 0.0f;
-    if (fl & F_ACTIVE) {
-        float sum = zero_in ? 0.0f : nbr_sum(pin, i, fl, g.px, g.plane);
-        float G = (sum - d[i]) / (float)__popc(fl & F_NBRS);
-        if (plain) {
-            out = G;
-        } else {
-            float pi = zero_in ? 0.0f : pin[i];
-            out = pi + omega * (G - pi);
+    float inv = cell_inv(flags[i]);
+    float sum = 0.0f, pi = 0.0f;
+    if (!zero_in) {
+        sum = sum6(pin[i - 1], pin[i + 1], pin[i - g.px], pin[i + g.px], pin[i - g.plane],
+                   pin[i + g.plane]);
+        pi = pin[i];
+    }
+    float G = (sum - d[i]) * inv;
+    float out = plain ? G : pi + omega * (G - pi);
+    pout[i] = (inv > 0.0f) ? out : 0.0f;
+}
+
+// T fused Jacobi sweeps (temporal blocking).  A TX x TY block owns one z-chunk
+// of a tile of columns; it computes the tile grown by T cells on each side in
+// x and y and writes only the inner (TX-2T) x (TY-2T) columns, whose values
+// after T sweeps depend only on data inside the grown tile.  Each thread holds
+// one column and streams it in z: level t (sweep t) at plane k-t is computed
+// from level t-1 at planes k-t-1, k-t, k-t+1 kept in registers and from the
+// four x/y neighbours of level t-1 at plane k-t held in LDS (double buffered,
+// one barrier per plane).  Every value is computed with the single-sweep
+// arithmetic, so results are bit-identical to T launches of k_jacobi.
+// The next plane is prefetched into registers one iteration ahead.
+template <int TX, int TY, int T, bool DAMPED>
+__global__ __launch_bounds__(TX * TY) void k_jacobi_fused(
+    Grid g, const float* __restrict__ pin, float* __restrict__ pout, const float* __restrict__ d,
+    const uint8_t* __restrict__ flags, int zc, int zero_in, float omega)
+{
+    // Padded by one on each side: edge threads read the (never written) pad,
+    // which only reaches halo columns that are not written out.
+    __shared__ float lds[T][2][TY + 2][TX + 2];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    constexpr int OX = TX - 2 * T, OY = TY - 2 * T;
+    const int x = (int)blockIdx.x * OX + tx - T + 1;
+    const int y = (int)blockIdx.y * OY + ty - T + 1;
+    const int z0 = (int)blockIdx.z * zc + 1;
+    const int z1 = min(z0 + zc, g.nz - 1);
+    const bool in_grid = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+    const bool writes = tx >= T && tx < TX - T && ty >= T && ty < TY - T && x <= g.nx - 2 &&
+                        y <= g.ny - 2;
+    const int base = in_grid ? g.idx(x, y, 0) : 0;
+
+    float w[T][3];  // level t at planes (newest-2, newest-1, newest)
+    float iq[T + 1], dq[T + 1];  // inverse and rhs of planes k, k-1, ..., k-T
+#pragma unroll
+    for (int t = 0; t < T; ++t) w[t][0] = w[t][1] = w[t][2] = 0.0f;
+#pragma unroll
+    for (int t = 0; t <= T; ++t) iq[t] = dq[t] = 0.0f;
+
+    float np = 0.0f, nd = 0.0f;
+    uint8_t nf = 0;
+    {
+        const int k = z0 - T;
+        if (in_grid && k >= 0 && k < g.nz) {
+            const int i = base + k * g.plane;
+            np = zero_in ? 0.0f : pin[i];
+            nf = flags[i];
+            nd = d[i];
         }
     }
-    pout[i] = out;
+    for (int k = z0 - T; k < z1 + T; ++k) {
+        const float p0 = np, dd = nd;
+        const uint8_t fl = nf;
+        {
+            const int kn = k + 1;
+            np = 0.0f;
+            nd = 0.0f;
+            nf = 0;
+            if (in_grid && kn >= 0 && kn < g.nz && kn < z1 + T) {
+                const int i = base + kn * g.plane;
+                np = zero_in ? 0.0f : pin[i];
+                nf = flags[i];
+                nd = d[i];
+            }
+        }
+#pragma unroll
+        for (int t = T; t > 0; --t) {
+            iq[t] = iq[t - 1];
+            dq[t] = dq[t - 1];
+        }
+        iq[0] = cell_inv(fl);
+        dq[0] = dd;
+        w[0][0] = w[0][1];
+        w[0][1] = w[0][2];
+        w[0][2] = p0;
+        const int cur = k & 1, prev = cur ^ 1;
+        float newv[T + 1];
+        newv[0] = p0;
+#pragma unroll
+        for (int t = 1; t <= T; ++t) {
+            const float sum = sum6(lds[t - 1][prev][ty + 1][tx], lds[t - 1][prev][ty + 1][tx + 2],
+                                   lds[t - 1][prev][ty][tx + 1], lds[t - 1][prev][ty + 2][tx + 1],
+                                   w[t - 1][0], w[t - 1][2]);
+            const float G = (sum - dq[t]) * iq[t];
+            float v;
+            if (DAMPED) {
+                const float pc = w[t - 1][1];
+                v = pc + omega * (G - pc);
+            } else {
+                v = G;
+            }
+            v = (iq[t] > 0.0f) ? v : 0.0f;
+            newv[t] = v;
+            if (t < T) {
+                w[t][0] = w[t][1];
+                w[t][1] = w[t][2];
+                w[t][2] = v;
+            }
+        }
+        const int zo = k - T;
+        if (writes && zo >= z0 && zo < z1) pout[base + zo * g.plane] = newv[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) lds[t][cur][ty + 1][tx + 1] = newv[t];
+        __syncthreads();
+    }
 }
 
 // One red-black SOR half-sweep of the pressure form, in place.
@@ -304,7 +440,7 @@ __global__ void k_rbsor_half(Grid g, float* __restrict__ p, const float* __restr
     int i = g.idx(x, y, z);
     uint8_t fl = flags[i];
     if (!(fl & F_ACTIVE)) return;
-    float G = (nbr_sum(p, i, fl, g.px, g.plane) - d[i]) / (float)__popc(fl & F_NBRS);
+    float G = (nbr_sum(p, i, fl, g.px, g.plane) - d[i]) * inv_acc(__popc(fl & F_NBRS));
     p[i] = p[i] + omega * (G - p[i]);
 }
 
@@ -596,6 +732,56 @@ hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const floa
     hipLaunchKernelGGL(k_jacobi, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
                        pin, pout, d, flags, omega, plain, zero_in);
     return hipGetLastError();
+}
+
+template <int TX, int TY, int T>
+static hipError_t launch_fused(const Grid& g, const float* pin, float* pout, const float* d,
+                               const uint8_t* flags, int zc, int zero_in, float omega, bool damped,
+                               hipStream_t st)
+{
+    constexpr int OX = TX - 2 * T, OY = TY - 2 * T;
+    const int m[3] = {g.nx - 2, g.ny - 2, g.nz - 2};
+    dim3 grid((unsigned)((m[0] + OX - 1) / OX), (unsigned)((m[1] + OY - 1) / OY),
+              (unsigned)((m[2] + zc - 1) / zc));
+    if (damped)
+        hipLaunchKernelGGL((k_jacobi_fused<TX, TY, T, true>), grid, dim3(TX, TY), 0, st, g, pin, pout,
+                           d, flags, zc, zero_in, omega);
+    else
+        hipLaunchKernelGGL((k_jacobi_fused<TX, TY, T, false>), grid, dim3(TX, TY), 0, st, g, pin,
+                           pout, d, flags, zc, zero_in, omega);
+    return hipGetLastError();
+}
+
+template <int TX, int TY>
+static hipError_t launch_fused_t(const Grid& g, const float* pin, float* pout, const float* d,
+                                 const uint8_t* flags, int t, int zc, int zero_in, float omega,
+                                 bool damped, hipStream_t st)
+{
+    switch (t) {
+    case 2: return launch_fused<TX, TY, 2>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 3: return launch_fused<TX, TY, 3>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 4: return launch_fused<TX, TY, 4>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 5: return launch_fused<TX, TY, 5>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 6: return launch_fused<TX, TY, 6>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t jacobi_fused(const Grid& g, const float* pin, float* pout, const float* d,
+                        const uint8_t* flags, const JacobiPlan& plan, int sweeps, float omega,
+                        int plain, int zero_in, hipStream_t st)
+{
+    if (sweeps == 1)
+        return jacobi_sweep(g, pin, pout, d, flags, omega, plain, zero_in, st);
+    if (plan.zc < 1) return hipErrorInvalidValue;
+    const bool damped = !plain;
+    switch (plan.tile) {
+    case JacobiPlan::TILE_32x32:
+        return launch_fused_t<32, 32>(g, pin, pout, d, flags, sweeps, plan.zc, zero_in, omega, damped, st);
+    case JacobiPlan::TILE_64x16:
+        return launch_fused_t<64, 16>(g, pin, pout, d, flags, sweeps, plan.zc, zero_in, omega, damped, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t rbsor_half(const Grid& g, float* p, const float* d, const uint8_t* flags, int colour,

@@ -54,6 +54,15 @@ class Params(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class JacobiPlan(ctypes.Structure):
+    _fields_ = [("sweeps_per_launch", ctypes.c_int), ("tile", ctypes.c_int), ("zc", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+TILE_32x32, TILE_64x16 = 0, 1
+
 _lib = None
 
 
@@ -79,6 +88,8 @@ def _declare(lib):
         "smoke_footprint_bytes": (ctypes.c_size_t, [u, u, u]),
         "smoke_set_params": (st, [vp, pp]),
         "smoke_get_params": (st, [vp, pp]),
+        "smoke_set_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
+        "smoke_get_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
         "smoke_add_obstacle": (st, [vp, f, f, f, f, f, f, f, ip]),
         "smoke_add_source": (st, [vp, f, f, f, f, ip]),
         "smoke_move_object": (st, [vp, i, f, f, f]),
@@ -217,6 +228,20 @@ class Simulation:
         cur.update(kw)
         p = Params(**cur)
         _check(self.lib.smoke_set_params(self.h, ctypes.byref(p)))
+
+    def get_jacobi_plan(self):
+        p = JacobiPlan()
+        _check(self.lib.smoke_get_jacobi_plan(self.h, ctypes.byref(p)))
+        return p.as_dict()
+
+    def set_jacobi_plan(self, **kw):
+        cur = self.get_jacobi_plan()
+        for k in kw:
+            if k not in cur:
+                raise KeyError(k)
+        cur.update(kw)
+        p = JacobiPlan(**cur)
+        _check(self.lib.smoke_set_jacobi_plan(self.h, ctypes.byref(p)))
 
     # scene
     def add_obstacle(self, x, y, z, r, vel=(0.0, 0.0, 0.0)):

@@ -137,3 +137,35 @@ def test_footprint_matches_allocation_budget():
     assert n > 0
     with smokesim.Simulation(64, 48, 40):
         pass
+
+
+PLANS = [(t, tile, zc) for t in (1, 2, 3, 4, 5, 6) for tile in (smokesim.TILE_32x32, smokesim.TILE_64x16)
+         for zc in (1, 7, 128)]
+
+
+@pytest.mark.parametrize("plan", PLANS, ids=lambda p: f"T{p[0]}-tile{p[1]}-zc{p[2]}")
+def test_every_jacobi_plan_is_bitwise_equal(plan):
+    """Fused Jacobi sweeps (temporal blocking) give the single-sweep bits for
+    every sweeps-per-launch, tile and z-chunk, plain and damped."""
+    t, tile, zc = plan
+    dims = (37, 29, 23)
+    sc = scenes.test_scene(dims, seed=21, projection=JAC, iterations=13, n_obstacles=2,
+                           n_sources=2)
+    sc["params"]["jacobi_omega"] = 0.9 if (t + zc) % 2 else 1.0
+    g = gpu_for(sc)
+    g.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
+    o = oracle_for(sc, 32)
+    for k in range(2):
+        g.step(sc["dt"], readback=False)
+        o.step(sc["dt"])
+    assert_bitwise_equal(g.state(), o.state(), f"plan {plan}")
+    p = g.get_pressure()
+    assert np.array_equal(p.view(np.uint32), o.get_pressure().astype(np.float32).view(np.uint32))
+
+
+def test_jacobi_plan_validation():
+    g = smokesim.Simulation(8, 8, 8)
+    for bad in (dict(sweeps_per_launch=0), dict(sweeps_per_launch=7), dict(tile=5), dict(zc=0)):
+        with pytest.raises(smokesim.SmokeError):
+            g.set_jacobi_plan(**bad)
+    assert g.get_jacobi_plan() == {"sweeps_per_launch": 4, "tile": 0, "zc": 128}
