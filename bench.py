@@ -107,6 +107,7 @@ def main():
     ap.add_argument("--projection", choices=["jacobi", "rbsor", "reference"], default=None)
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default Jacobi plan")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     args = ap.parse_args()
 
@@ -132,6 +133,11 @@ def main():
     sim = smokesim.Simulation(nx, ny, nz)
     smokesim.scenes.apply(sim, sc)
     dt = sc["dt"]
+    tuned = None
+    if sc["params"]["projection"] == 2 and not args.no_autotune:
+        plan, ms = sim.autotune()
+        tuned = {"plan": plan, "ms_per_sweep": round(ms, 5)}
+        log(f"autotune: {plan} at {ms * 1e3:.2f} us/sweep")
     for _ in range(args.warmup):
         sim.step_async(dt)
     sim.synchronize()
@@ -184,6 +190,7 @@ def main():
             "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2),
             "alg_bytes_per_launch": int(alg), "launches_timed": sweep_launches,
             "jacobi_plan": sim.get_jacobi_plan() if p["projection"] == 2 else None,
+            "autotune": tuned,
         }
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if roofline and os.path.exists(traffic_file):

@@ -130,10 +130,19 @@ typedef enum smoke_tile { SMOKE_TILE_32x32 = 0, SMOKE_TILE_64x16 = 1 } smoke_til
 typedef struct smoke_jacobi_plan {
     int sweeps_per_launch; /* 1..6 (default 4) */
     int tile;              /* smoke_tile (default 32x32) */
-    int zc;                /* z planes per block, >= 1 (default 128) */
+    int zc;                /* z planes per block, >= 1 (default 64) */
 } smoke_jacobi_plan;
 smoke_status smoke_set_jacobi_plan(smoke_sim* sim, const smoke_jacobi_plan* plan);
 smoke_status smoke_get_jacobi_plan(const smoke_sim* sim, smoke_jacobi_plan* out);
+
+/* Autotuner: times the projection's sweep sequence (the simulation's current
+ * iteration count, or 100 if that is 0) on this simulation's buffers for
+ * every candidate plan -- 2..6 sweeps per launch x both tiles x z-chunks of
+ * 32, 64 and 128 planes -- and keeps the fastest.  Results are identical for
+ * every plan, so tuning never changes answers.  Resets the pressure kept for
+ * warm starts to zero.  `chosen` (may be NULL) receives the plan and
+ * `ms_per_sweep` (may be NULL) its measured time per sweep. */
+smoke_status smoke_autotune(smoke_sim* sim, smoke_jacobi_plan* chosen, double* ms_per_sweep);
 
 /* ---------------------------------------------------------------- scene */
 

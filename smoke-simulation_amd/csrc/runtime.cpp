@@ -212,6 +212,27 @@ hipError_t stage_force(smoke_sim* s, float dt, long long* launches)
     return smk::force(s->g, s->u[n], s->v[n], s->w[n], s->rho[n], s->flags, a, s->stream);
 }
 
+// K Jacobi sweeps as launches of plan.sweeps_per_launch fused sweeps each, the
+// remainder in one shorter launch; ping-pongs the two P buffers.
+hipError_t enqueue_jacobi(smoke_sim* s, const smk::JacobiPlan& plan, int sweeps, bool zero_in,
+                          long long* launches)
+{
+    const bool plain = s->prm.jacobi_omega == 1.0;
+    for (int left = sweeps; left > 0;) {
+        const int t = std::min(left, plan.sweeps_per_launch);
+        const int out = 1 - s->pcur;
+        hipError_t e = smk::jacobi_fused(s->g, s->P[s->pcur], s->P[out], s->d, s->flags, plan, t,
+                                         (float)s->prm.jacobi_omega, plain ? 1 : 0, zero_in ? 1 : 0,
+                                         s->stream);
+        if (e != hipSuccess) return e;
+        *launches += 1;
+        s->pcur = out;
+        zero_in = false;
+        left -= t;
+    }
+    return hipSuccess;
+}
+
 // Records the begin (0) or end (1) event of the sweep loop of a timed step.
 hipError_t sweep_mark(smoke_sim* s, int which, long long sweeps)
 {
@@ -241,8 +262,7 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
     if (e != hipSuccess) return e;
     *launches += 1;
     if (p.projection == SMOKE_PROJ_JACOBI) {
-        const bool plain = p.jacobi_omega == 1.0;
-        bool zero_in = !p.warm_start;
+        const bool zero_in = !p.warm_start;
         if (p.iterations == 0 && zero_in) {  // P = 0: no correction
             if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
             return sweep_mark(s, 1, 0);
@@ -253,21 +273,10 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
             *launches += 1;
         }
         if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
-        // K sweeps as launches of plan.sweeps_per_launch fused sweeps each,
-        // the remainder in one shorter launch.
         long long sweep_launches = 0;
-        for (int left = p.iterations; left > 0;) {
-            const int t = std::min(left, s->plan.sweeps_per_launch);
-            const int out = 1 - s->pcur;
-            e = smk::jacobi_fused(g, s->P[s->pcur], s->P[out], s->d, s->flags, s->plan, t,
-                                  (float)p.jacobi_omega, plain ? 1 : 0, zero_in ? 1 : 0, s->stream);
-            if (e != hipSuccess) return e;
-            *launches += 1;
-            sweep_launches += 1;
-            s->pcur = out;
-            zero_in = false;
-            left -= t;
-        }
+        e = enqueue_jacobi(s, s->plan, p.iterations, zero_in, &sweep_launches);
+        if (e != hipSuccess) return e;
+        *launches += sweep_launches;
         if ((e = sweep_mark(s, 1, sweep_launches)) != hipSuccess) return e;
     } else {
         if (p.warm_start) e = smk::mask_pressure(g, s->P[s->pcur], s->flags, s->stream);
@@ -566,6 +575,66 @@ smoke_status smoke_get_jacobi_plan(const smoke_sim* s, smoke_jacobi_plan* out)
     out->sweeps_per_launch = s->plan.sweeps_per_launch;
     out->tile = s->plan.tile;
     out->zc = s->plan.zc;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_autotune(smoke_sim* s, smoke_jacobi_plan* chosen, double* ms_per_sweep)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "autotune: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int K = s->prm.iterations > 0 ? s->prm.iterations : 100;
+    HIP_TRY(refresh_mask(s, nullptr));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
+        cleanup();
+        return fail(SMOKE_ERR_HIP, "autotune: cannot create events");
+    }
+    smk::JacobiPlan best = s->plan;
+    double best_ms = -1.0;
+    const int tiles[2] = {smk::JacobiPlan::TILE_32x32, smk::JacobiPlan::TILE_64x16};
+    const int zcs[3] = {32, 64, 128};
+    constexpr int kReps = 2;
+    hipError_t e = hipSuccess;
+    for (int t = 2; t <= 6 && e == hipSuccess; ++t)
+        for (int tile : tiles)
+            for (int zc : zcs) {
+                smk::JacobiPlan cand;
+                cand.sweeps_per_launch = t;
+                cand.tile = tile;
+                cand.zc = zc;
+                long long n = 0;
+                e = enqueue_jacobi(s, cand, K, false, &n);  // warm-up
+                if (e == hipSuccess) e = hipEventRecord(ev[0], s->stream);
+                for (int r = 0; r < kReps && e == hipSuccess; ++r) e = enqueue_jacobi(s, cand, K, false, &n);
+                if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (e != hipSuccess) break;
+                const double per_sweep = ms / (double)(kReps * K);
+                if (best_ms < 0.0 || per_sweep < best_ms) {
+                    best_ms = per_sweep;
+                    best = cand;
+                }
+            }
+    // The sweeps ran on whatever right-hand side the buffer held: start the
+    // next warm start from zero.
+    if (e == hipSuccess) e = smk::fill_f32(s->P[0], s->g.elems, 0.0f, s->stream);
+    if (e == hipSuccess) e = smk::fill_f32(s->P[1], s->g.elems, 0.0f, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(SMOKE_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+    s->plan = best;
+    if (chosen) {
+        chosen->sweeps_per_launch = best.sweeps_per_launch;
+        chosen->tile = best.tile;
+        chosen->zc = best.zc;
+    }
+    if (ms_per_sweep) *ms_per_sweep = best_ms;
     return SMOKE_OK;
 }
 

@@ -59,7 +59,7 @@ struct JacobiPlan {
     enum Tile { TILE_32x32 = 0, TILE_64x16 = 1 };
     int sweeps_per_launch = 4;
     int tile = TILE_32x32;
-    int zc = 128;
+    int zc = 64;
 };
 
 // ------------------------------------------------------------ launchers

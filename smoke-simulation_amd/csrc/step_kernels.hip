@@ -343,9 +343,19 @@ __global__ __launch_bounds__(TX * TY) void k_jacobi_fused(
     __shared__ float lds[T][2][TY + 2][TX + 2];
     const int tx = threadIdx.x, ty = threadIdx.y;
     constexpr int OX = TX - 2 * T, OY = TY - 2 * T;
-    const int x = (int)blockIdx.x * OX + tx - T + 1;
-    const int y = (int)blockIdx.y * OY + ty - T + 1;
-    const int z0 = (int)blockIdx.z * zc + 1;
+    // Workgroup b runs on XCD b % 8 (8 XCDs, each with its own L2).  Renumber
+    // so that each XCD gets one contiguous run of tiles (x fastest): tiles
+    // that share halo columns then run on the same L2 at the same time.  The
+    // renumbering is a permutation of the tiles (the tail beyond a multiple
+    // of 8 keeps its index).
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int nb = nbx * nby * (int)gridDim.z;
+    const int b = (int)blockIdx.x + nbx * ((int)blockIdx.y + nby * (int)blockIdx.z);
+    const int full = nb & ~7;
+    const int L = (b < full) ? (b & 7) * (full >> 3) + (b >> 3) : b;
+    const int x = (L % nbx) * OX + tx - T + 1;
+    const int y = ((L / nbx) % nby) * OY + ty - T + 1;
+    const int z0 = (L / (nbx * nby)) * zc + 1;
     const int z1 = min(z0 + zc, g.nz - 1);
     const bool in_grid = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
     const bool writes = tx >= T && tx < TX - T && ty >= T && ty < TY - T && x <= g.nx - 2 &&

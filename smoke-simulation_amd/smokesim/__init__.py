@@ -90,6 +90,7 @@ def _declare(lib):
         "smoke_get_params": (st, [vp, pp]),
         "smoke_set_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
         "smoke_get_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
+        "smoke_autotune": (st, [vp, ctypes.POINTER(JacobiPlan), dp]),
         "smoke_add_obstacle": (st, [vp, f, f, f, f, f, f, f, ip]),
         "smoke_add_source": (st, [vp, f, f, f, f, ip]),
         "smoke_move_object": (st, [vp, i, f, f, f]),
@@ -242,6 +243,14 @@ class Simulation:
         cur.update(kw)
         p = JacobiPlan(**cur)
         _check(self.lib.smoke_set_jacobi_plan(self.h, ctypes.byref(p)))
+
+    def autotune(self):
+        """Pick the fastest fused-Jacobi plan for this grid and iteration
+        count; returns (plan dict, ms per sweep)."""
+        p = JacobiPlan()
+        ms = ctypes.c_double()
+        _check(self.lib.smoke_autotune(self.h, ctypes.byref(p), ctypes.byref(ms)))
+        return p.as_dict(), ms.value
 
     # scene
     def add_obstacle(self, x, y, z, r, vel=(0.0, 0.0, 0.0)):

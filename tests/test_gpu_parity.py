@@ -168,4 +168,19 @@ def test_jacobi_plan_validation():
     for bad in (dict(sweeps_per_launch=0), dict(sweeps_per_launch=7), dict(tile=5), dict(zc=0)):
         with pytest.raises(smokesim.SmokeError):
             g.set_jacobi_plan(**bad)
-    assert g.get_jacobi_plan() == {"sweeps_per_launch": 4, "tile": 0, "zc": 128}
+    assert g.get_jacobi_plan() == {"sweeps_per_launch": 4, "tile": 0, "zc": 64}
+
+
+def test_autotune_picks_a_plan_and_keeps_results_bitwise():
+    dims = (48, 40, 36)
+    sc = scenes.test_scene(dims, seed=22, projection=JAC, iterations=17, n_obstacles=1,
+                           n_sources=2)
+    g = gpu_for(sc)
+    plan, ms = g.autotune()
+    assert 1 <= plan["sweeps_per_launch"] <= 6 and ms > 0.0
+    assert g.get_jacobi_plan() == plan
+    o = oracle_for(sc, 32)
+    for _ in range(2):
+        g.step(sc["dt"], readback=False)
+        o.step(sc["dt"])
+    assert_bitwise_equal(g.state(), o.state(), f"autotuned plan {plan}")
