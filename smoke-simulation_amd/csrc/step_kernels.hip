@@ -507,35 +507,77 @@ __device__ __forceinline__ float avgW(const float* __restrict__ a, int i, int px
     return t / 8.0f;
 }
 
-// sampleSmoke (smokeSimulation.cu:451-484): clamp to [1, n-1] with the cell
-// dims, trilinear weights, fixed corner order.
-__device__ __forceinline__ float sample(const Grid& g, const float* __restrict__ a, float px_,
-                                        float py_, float pz_, float dx, float dy, float dz)
+// sampleSmoke (smokeSimulation.cu:451-484), split in two so that a kernel can
+// issue the loads of several samples together: Tri holds the clamped
+// trilinear weights and the four row offsets of the corner pairs (x0, x0+1).
+struct Tri {
+    float xw0, xw1, yw0, yw1, zw0, zw1;
+    int r00, r10, r01, r11;  // idx(x0, y0|y1, z0|z1): first index y, second z
+    bool xsame;              // x1 == x0 (position clamped at the last cell)
+};
+
+// Clamp to [1, n-1] with the cell dims and form weights and indices.
+__device__ __forceinline__ Tri tri_setup(const Grid& g, float px_, float py_, float pz_, float dx,
+                                         float dy, float dz)
 {
+    Tri t;
     float x = fmaxf(fminf(px_, (float)(g.nx - 1)), 1.0f);
     float y = fmaxf(fminf(py_, (float)(g.ny - 1)), 1.0f);
     float z = fmaxf(fminf(pz_, (float)(g.nz - 1)), 1.0f);
     int x0 = (int)fminf(floorf(x - dx), (float)(g.nx - 1));
-    float xw1 = (x - dx) - (float)x0;
-    float xw0 = 1.0f - xw1;
+    t.xw1 = (x - dx) - (float)x0;
+    t.xw0 = 1.0f - t.xw1;
     int x1 = (int)fminf((float)(x0 + 1), (float)(g.nx - 1));
     int y0 = (int)fminf(floorf(y - dy), (float)(g.ny - 1));
-    float yw1 = (y - dy) - (float)y0;
-    float yw0 = 1.0f - yw1;
+    t.yw1 = (y - dy) - (float)y0;
+    t.yw0 = 1.0f - t.yw1;
     int y1 = (int)fminf((float)(y0 + 1), (float)(g.ny - 1));
     int z0 = (int)fminf(floorf(z - dz), (float)(g.nz - 1));
-    float zw1 = (z - dz) - (float)z0;
-    float zw0 = 1.0f - zw1;
+    t.zw1 = (z - dz) - (float)z0;
+    t.zw0 = 1.0f - t.zw1;
     int z1 = (int)fminf((float)(z0 + 1), (float)(g.nz - 1));
-    return xw0 * yw0 * zw0 * a[g.idx(x0, y0, z0)] + xw1 * yw0 * zw0 * a[g.idx(x1, y0, z0)] +
-           xw0 * yw1 * zw0 * a[g.idx(x0, y1, z0)] + xw1 * yw1 * zw0 * a[g.idx(x1, y1, z0)] +
-           xw0 * yw0 * zw1 * a[g.idx(x0, y0, z1)] + xw1 * yw0 * zw1 * a[g.idx(x1, y0, z1)] +
-           xw0 * yw1 * zw1 * a[g.idx(x0, y1, z1)] + xw1 * yw1 * zw1 * a[g.idx(x1, y1, z1)];
+    t.r00 = g.idx(x0, y0, z0);
+    t.r10 = g.idx(x0, y1, z0);
+    t.r01 = g.idx(x0, y0, z1);
+    t.r11 = g.idx(x0, y1, z1);
+    t.xsame = x1 == x0;
+    return t;
+}
+
+// Two consecutive floats from a 4-byte-aligned address (one dwordx2 load).
+// a[r+1] exists for every row offset r of a cell x0 <= nx-1: rows are
+// px >= nx+1 long.
+__device__ __forceinline__ float2 ld2(const float* __restrict__ a, int r)
+{
+    float2 v;
+    __builtin_memcpy(&v, a + r, sizeof v);
+    return v;
+}
+
+// The eight corners in the reference's order (:473-481).
+__device__ __forceinline__ float tri_eval(const float* __restrict__ a, const Tri& t)
+{
+    const float2 p00 = ld2(a, t.r00), p10 = ld2(a, t.r10), p01 = ld2(a, t.r01),
+                 p11 = ld2(a, t.r11);
+    const float f000 = p00.x, f100 = t.xsame ? p00.x : p00.y;
+    const float f010 = p10.x, f110 = t.xsame ? p10.x : p10.y;
+    const float f001 = p01.x, f101 = t.xsame ? p01.x : p01.y;
+    const float f011 = p11.x, f111 = t.xsame ? p11.x : p11.y;
+    return t.xw0 * t.yw0 * t.zw0 * f000 + t.xw1 * t.yw0 * t.zw0 * f100 +
+           t.xw0 * t.yw1 * t.zw0 * f010 + t.xw1 * t.yw1 * t.zw0 * f110 +
+           t.xw0 * t.yw0 * t.zw1 * f001 + t.xw1 * t.yw0 * t.zw1 * f101 +
+           t.xw0 * t.yw1 * t.zw1 * f011 + t.xw1 * t.yw1 * t.zw1 * f111;
 }
 
 // velocityAdvectionU/V/W (smokeSimulation.cu:527-615) fused per staggered
 // index over x,y,z in [1,n-1]: each component keeps its own range and
-// fluid-fluid condition; read [now], write [past].
+// fluid-fluid condition; read [now], write [past].  The three averages are
+// shared between the components that use them (avgW feeds both U and V with
+// the same arguments, and so on), and all loads of a phase are issued
+// together: first the own components and the averages, then the corners of
+// all three samples.  Every address read lies inside the allocation (faces up
+// to x = nx, y = ny; clamped sample corners), so the loads are unconditional
+// and only the stores are predicated.
 __global__ void k_advect_velocity(Grid g, const float* __restrict__ u0, const float* __restrict__ v0,
                                   const float* __restrict__ w0, float* __restrict__ u1,
                                   float* __restrict__ v1, float* __restrict__ w1,
@@ -547,38 +589,29 @@ __global__ void k_advect_velocity(Grid g, const float* __restrict__ u0, const fl
     if (x > g.nx - 1 || y > g.ny - 1 || z > g.nz - 1) return;
     int i = g.idx(x, y, z);
     uint8_t fl = flags[i];
-    if (!(fl & F_SELF)) return;
+    const bool self = (fl & F_SELF) != 0;
+    const bool doU = self && y < g.ny - 1 && z < g.nz - 1 && (fl & F_XM);
+    const bool doV = self && x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM);
+    const bool doW = self && x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM);
+    if (!(doU || doV || doW)) return;
     const int px = g.px, pl = g.plane;
-    if (y < g.ny - 1 && z < g.nz - 1 && (fl & F_XM)) {
-        float xt = (float)x, yt = (float)y + 0.5f, zt = (float)z + 0.5f;
-        float uu = u0[i];
-        float vv = avgV(v0, i, px, pl);
-        float ww = avgW(w0, i, px, pl);
-        xt = xt - dt * uu;
-        yt = yt - dt * vv;
-        zt = zt - dt * ww;
-        u1[i] = sample(g, u0, xt, yt, zt, 0.0f, 0.5f, 0.5f);
-    }
-    if (x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM)) {
-        float xt = (float)x + 0.5f, yt = (float)y, zt = (float)z + 0.5f;
-        float uu = avgU(u0, i, px, pl);
-        float vv = v0[i];
-        float ww = avgW(w0, i, px, pl);
-        xt = xt - dt * uu;
-        yt = yt - dt * vv;
-        zt = zt - dt * ww;
-        v1[i] = sample(g, v0, xt, yt, zt, 0.5f, 0.0f, 0.5f);
-    }
-    if (x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM)) {
-        float xt = (float)x + 0.5f, yt = (float)y + 0.5f, zt = (float)z;
-        float uu = avgU(u0, i, px, pl);
-        float vv = avgV(v0, i, px, pl);
-        float ww = w0[i];
-        xt = xt - dt * uu;
-        yt = yt - dt * vv;
-        zt = zt - dt * ww;
-        w1[i] = sample(g, w0, xt, yt, zt, 0.5f, 0.5f, 0.0f);
-    }
+    const float uu = u0[i], vv = v0[i], ww = w0[i];
+    const float aU = avgU(u0, i, px, pl);
+    const float aV = avgV(v0, i, px, pl);
+    const float aW = avgW(w0, i, px, pl);
+    const float xf = (float)x, yf = (float)y, zf = (float)z;
+    const Tri tu = tri_setup(g, xf - dt * uu, (yf + 0.5f) - dt * aV, (zf + 0.5f) - dt * aW, 0.0f,
+                             0.5f, 0.5f);
+    const Tri tv = tri_setup(g, (xf + 0.5f) - dt * aU, yf - dt * vv, (zf + 0.5f) - dt * aW, 0.5f,
+                             0.0f, 0.5f);
+    const Tri tw = tri_setup(g, (xf + 0.5f) - dt * aU, (yf + 0.5f) - dt * aV, zf - dt * ww, 0.5f,
+                             0.5f, 0.0f);
+    const float nu = tri_eval(u0, tu);
+    const float nv = tri_eval(v0, tv);
+    const float nw = tri_eval(w0, tw);
+    if (doU) u1[i] = nu;
+    if (doV) v1[i] = nv;
+    if (doW) w1[i] = nw;
 }
 
 // advectSmoke (smokeSimulation.cu:617-638): cell velocity from the
@@ -600,7 +633,7 @@ __global__ void k_advect_density(Grid g, const float* __restrict__ r0, float* __
     float xt = (float)((double)x + 0.5 - (double)(ut * dt));
     float yt = (float)((double)y + 0.5 - (double)(vt * dt));
     float zt = (float)((double)z + 0.5 - (double)(wt * dt));
-    r1[i] = sample(g, r0, xt, yt, zt, 0.5f, 0.5f, 0.5f);
+    r1[i] = tri_eval(r0, tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f));
 }
 
 }  // namespace
