@@ -209,6 +209,28 @@ smoke_status smoke_read_density(smoke_sim* sim, float* dst);
 /* Waits for all work enqueued on the simulation's stream. */
 smoke_status smoke_synchronize(smoke_sim* sim);
 
+/* ----------------------------------------------- asynchronous read-back */
+/* The reference copies the density to the host synchronously after every
+ * step (smokeSimulation.cu:814) and names that copy a bottleneck.  These two
+ * calls overlap it with the following steps: begin() packs the density of the
+ * last enqueued step on the device (the next step waits only for this pack)
+ * and starts its copy into one of two pinned host buffers (used in turn);
+ * wait() blocks until the oldest started copy has landed and returns its
+ * buffer, which stays valid until the second begin() after the one that
+ * produced it.  At most two read-backs may be pending: a third begin()
+ * returns SMOKE_ERR_STATE until wait() is called.  Formats: FP32, or FP16 /
+ * BF16 (half the bytes; round-to-nearest-even conversion of the FP32
+ * density). */
+typedef enum smoke_readback_format {
+    SMOKE_READBACK_F32 = 0,
+    SMOKE_READBACK_F16 = 1,
+    SMOKE_READBACK_BF16 = 2
+} smoke_readback_format;
+smoke_status smoke_readback_begin(smoke_sim* sim, int format);
+/* *host: nx*ny*nz values of the chosen format, x fastest; *step: 1-based
+ * number of the step the density belongs to. */
+smoke_status smoke_readback_wait(smoke_sim* sim, const void** host, long long* step);
+
 /* ---------------------------------------------------------------- timing */
 
 /* Per-stage device timing with HIP events recorded on the simulation's

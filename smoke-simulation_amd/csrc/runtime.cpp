@@ -126,6 +126,21 @@ struct smoke_sim {
     int pcur = 0;  // P buffer holding the latest pressure
     bool mask_dirty = true;
     smk::JacobiPlan plan;  // launch plan of the fused Jacobi sweeps
+    long long steps_enqueued = 0;
+
+    // Asynchronous read-back: a copy stream, two device pack slots and two
+    // pinned host buffers (allocated on first use).
+    struct Readback {
+        int slot;
+        long long step;
+    };
+    hipStream_t copy_stream = nullptr;
+    void* rb_dev = nullptr;          // 2 slots of nx*ny*nz*4 bytes
+    void* rb_host[2] = {nullptr, nullptr};
+    hipEvent_t rb_step_done = nullptr, rb_packed[2] = {nullptr, nullptr},
+               rb_copied[2] = {nullptr, nullptr};
+    std::deque<Readback> rb_pending;
+    int rb_next = 0;
 
     // Timing: events at every stage boundary of a step, plus two around the
     // projection's sweep launches (the step's dominant kernel).
@@ -397,6 +412,7 @@ smoke_status step_locked(smoke_sim* s, float dt)
         }
     }
     if (s->timing) s->pending.push_back(se);
+    s->steps_enqueued += 1;
     s->cur = nullptr;
     return SMOKE_OK;
 }
@@ -528,9 +544,84 @@ void smoke_destroy(smoke_sim* s)
         (void)hipEventDestroy(se.sweep_ev[1]);
     }
     for (hipEvent_t ev : s->event_pool) (void)hipEventDestroy(ev);
+    if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
+    hipEvent_t rb_events[5] = {s->rb_step_done, s->rb_packed[0], s->rb_packed[1], s->rb_copied[0],
+                               s->rb_copied[1]};
+    for (hipEvent_t ev : rb_events)
+        if (ev) (void)hipEventDestroy(ev);
+    for (void* h : s->rb_host)
+        if (h) (void)hipHostFree(h);
+    if (s->rb_dev) (void)hipFree(s->rb_dev);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+}
+
+smoke_status smoke_readback_begin(smoke_sim* s, int format)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "readback_begin: NULL sim");
+    if (format < SMOKE_READBACK_F32 || format > SMOKE_READBACK_BF16)
+        return fail(SMOKE_ERR_INVALID, "readback_begin: unknown format %d", format);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->rb_pending.size() >= 2)
+        return fail(SMOKE_ERR_STATE, "readback_begin: two read-backs pending; call smoke_readback_wait");
+    const size_t cells = (size_t)s->g.nx * s->g.ny * s->g.nz;
+    const size_t slot_bytes = cells * sizeof(float);
+    if (!s->copy_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&s->rb_step_done, hipEventDisableTiming));
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(hipEventCreateWithFlags(&s->rb_packed[k], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&s->rb_copied[k], hipEventDisableTiming));
+        }
+    }
+    if (!s->rb_dev) {
+        if (hipMalloc(&s->rb_dev, 2 * slot_bytes) != hipSuccess) {
+            s->rb_dev = nullptr;
+            return fail(SMOKE_ERR_NOMEM, "readback_begin: cannot allocate device slots");
+        }
+        for (int k = 0; k < 2; ++k)
+            if (hipHostMalloc(&s->rb_host[k], slot_bytes, hipHostMallocDefault) != hipSuccess) {
+                s->rb_host[k] = nullptr;
+                return fail(SMOKE_ERR_NOMEM, "readback_begin: cannot allocate pinned host buffers");
+            }
+    }
+    const int slot = s->rb_next;
+    char* dev = static_cast<char*>(s->rb_dev) + slot * slot_bytes;
+    const float* src = s->rho[1 - s->now];
+    // The copy stream packs after the last enqueued step; the next step's
+    // scene stage (which writes sources into both density halves) waits for
+    // the pack, not for the host copy.
+    HIP_TRY(hipEventRecord(s->rb_step_done, s->stream));
+    HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->rb_step_done, 0));
+    size_t bytes = slot_bytes;
+    if (format == SMOKE_READBACK_F32) {
+        HIP_TRY(smk::pack_cells_f32(s->g, src, reinterpret_cast<float*>(dev), s->copy_stream));
+    } else {
+        HIP_TRY(smk::pack_cells_half(s->g, src, reinterpret_cast<uint16_t*>(dev), format, s->copy_stream));
+        bytes = cells * sizeof(uint16_t);
+    }
+    HIP_TRY(hipEventRecord(s->rb_packed[slot], s->copy_stream));
+    HIP_TRY(hipStreamWaitEvent(s->stream, s->rb_packed[slot], 0));
+    HIP_TRY(hipMemcpyAsync(s->rb_host[slot], dev, bytes, hipMemcpyDeviceToHost, s->copy_stream));
+    HIP_TRY(hipEventRecord(s->rb_copied[slot], s->copy_stream));
+    s->rb_pending.push_back({slot, s->steps_enqueued});
+    s->rb_next = 1 - slot;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_readback_wait(smoke_sim* s, const void** host, long long* step)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "readback_wait: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->rb_pending.empty()) return fail(SMOKE_ERR_STATE, "readback_wait: nothing pending");
+    const smoke_sim::Readback r = s->rb_pending.front();
+    HIP_TRY(hipEventSynchronize(s->rb_copied[r.slot]));
+    s->rb_pending.pop_front();
+    if (host) *host = s->rb_host[r.slot];
+    if (step) *step = r.step;
+    return SMOKE_OK;
 }
 
 smoke_status smoke_get_dims(const smoke_sim* s, unsigned* nx, unsigned* ny, unsigned* nz)

@@ -75,6 +75,9 @@ hipError_t unpack_faces_f32(const Grid& g, const float* src, float* dst, hipStre
 hipError_t pack_faces_f32(const Grid& g, const float* src, float* dst, hipStream_t st);
 hipError_t unpack_cells_solid(const Grid& g, const uint8_t* solid, uint8_t* base, hipStream_t st);
 hipError_t pack_cells_u8(const Grid& g, const uint8_t* src, uint8_t* dst, hipStream_t st);
+// Density to contiguous FP16 (format 1) or BF16 (format 2), round to nearest even.
+hipError_t pack_cells_half(const Grid& g, const float* src, uint16_t* dst, int format,
+                           hipStream_t st);
 hipError_t floor_mask(const Grid& g, uint8_t* base, hipStream_t st);
 
 // Scene: solid mask from base + obstacles, then flags; sources.

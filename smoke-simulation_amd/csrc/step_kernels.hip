@@ -60,6 +60,29 @@ __global__ void k_repack(Grid g, const T* __restrict__ src, T* __restrict__ dst,
     else dst[c] = src[i];
 }
 
+// Padded FP32 cells -> contiguous 16-bit values: format 1 FP16 (hardware
+// conversion, round to nearest even), format 2 BF16 (round to nearest even on
+// the FP32 bits; NaN stays a quiet NaN).
+__global__ void k_pack_half(Grid g, const float* __restrict__ src, uint16_t* __restrict__ dst,
+                            int format)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    const float f = src[g.idx(x, y, z)];
+    uint16_t h;
+    if (format == 1) {
+        const _Float16 v = (_Float16)f;
+        h = __builtin_bit_cast(uint16_t, v);
+    } else {
+        const uint32_t b = __float_as_uint(f);
+        if ((b & 0x7fffffffu) > 0x7f800000u) h = (uint16_t)((b >> 16) | 0x0040u);
+        else h = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+    }
+    dst[cidx(x, y, z, g.nx, g.ny)] = h;
+}
+
 __global__ void k_unpack_solid(Grid g, const uint8_t* __restrict__ solid, uint8_t* __restrict__ base)
 {
     int x = blockIdx.x * BX + threadIdx.x;
@@ -697,6 +720,14 @@ hipError_t pack_cells_u8(const Grid& g, const uint8_t* src, uint8_t* dst, hipStr
 {
     hipLaunchKernelGGL((k_repack<false, uint8_t>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,
                        g, src, dst, g.nx, g.ny, g.nz);
+    return hipGetLastError();
+}
+
+hipError_t pack_cells_half(const Grid& g, const float* src, uint16_t* dst, int format,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_half, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, src, dst,
+                       format);
     return hipGetLastError();
 }
 

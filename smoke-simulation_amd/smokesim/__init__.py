@@ -26,6 +26,7 @@ STAGE_SCENE, STAGE_FORCE, STAGE_PROJECT, STAGE_ADVECT_VEL, STAGE_ADVECT_DENS = r
 STAGE_COUNT = 5
 STAGE_NAMES = ("scene", "force", "project", "advect_velocity", "advect_density")
 MAX_OBJECTS = 64
+READBACK_F32, READBACK_F16, READBACK_BF16 = 0, 1, 2
 
 F_SELF, F_XM, F_XP, F_YM, F_YP, F_ZM, F_ZP, F_ACTIVE = 1, 2, 4, 8, 16, 32, 64, 128
 
@@ -108,6 +109,8 @@ def _declare(lib):
         "smoke_run_stage": (st, [vp, i, f]),
         "smoke_read_density": (st, [vp, fp]),
         "smoke_synchronize": (st, [vp]),
+        "smoke_readback_begin": (st, [vp, i]),
+        "smoke_readback_wait": (st, [vp, ctypes.POINTER(vp), llp]),
         "smoke_timing_enable": (st, [vp, i]),
         "smoke_timing_read": (st, [vp, dp, llp, llp]),
         "smoke_timing_reset": (st, [vp]),
@@ -331,6 +334,25 @@ class Simulation:
 
     def synchronize(self):
         _check(self.lib.smoke_synchronize(self.h))
+
+    def readback_begin(self, fmt=READBACK_F32):
+        _check(self.lib.smoke_readback_begin(self.h, int(fmt)))
+        self._rb_formats = getattr(self, "_rb_formats", []) + [int(fmt)]
+
+    def readback_wait(self):
+        """(density array copied out of the pinned buffer, step number)."""
+        ptr = ctypes.c_void_p()
+        step = ctypes.c_longlong()
+        _check(self.lib.smoke_readback_wait(self.h, ctypes.byref(ptr), ctypes.byref(step)))
+        fmt = self._rb_formats.pop(0)
+        n = self.cells
+        if fmt == READBACK_F32:
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), (n,))
+        else:
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint16)), (n,))
+            if fmt == READBACK_F16:
+                a = a.view(np.float16)
+        return a.copy().reshape(self.cell_shape), step.value
 
     # timing
     def timing_enable(self, on=True):
