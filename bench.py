@@ -9,6 +9,11 @@ the timed region; the timed region is K full steps (sources, mask, buoyancy,
 clamp, divergence, 100 sweeps, gradient, velocity and density advection)
 enqueued back to back and bracketed by stream synchronisations.
 
+Other workloads: grid_512 (512^3, 8 sources), reference_80 (the reference's
+own 80^3 scene, red-black SOR on faces), batch_64 (64 simulations of uneven
+sizes in [32,160]^3, per-simulation dt, Jacobi x60, one launch per stage,
+replayed from a HIP graph).
+
 Prints ONE JSON line on stdout (metric, value, roofline, cpu_baseline, ...);
 progress goes to stderr.
 """
@@ -34,21 +39,11 @@ BYTES_ADVECT_DENS = 21         # rho, u, v, w, flags in; rho out
 BYTES_REF_HALF = 25            # in-place half-sweep on faces: u, v, w, flags in; u, v, w out
 BYTES_FIXED = (BYTES_FORCE + BYTES_DIVERGENCE + BYTES_GRADIENT + BYTES_ADVECT_VEL
                + BYTES_ADVECT_DENS)     # 121
+METRIC = "grid cell-updates/s per full step"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def workload(name):
-    from smokesim import scenes
-    if name == "grid_256":
-        return scenes.grid_256()
-    if name == "grid_512":
-        return scenes.grid_512()
-    if name == "reference_80":
-        return scenes.reference_80()
-    raise SystemExit(f"unknown workload {name}")
 
 
 def describe(sc):
@@ -61,7 +56,7 @@ def describe(sc):
             f"{len(sc['sources'])} source(s), {proj} x{p['iterations']}, dt={sc['dt']}")
 
 
-def sweep_bytes_per_cell(params):
+def step_bytes_per_cell(params):
     """Algorithmic bytes per cell of one full step for the chosen projection."""
     k = params["iterations"]
     if params["projection"] == 0:
@@ -98,30 +93,15 @@ def cpu_baseline(sc, budget_s, max_steps):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="grid_256")
-    ap.add_argument("--projection", choices=["jacobi", "rbsor", "reference"], default=None)
-    ap.add_argument("--iterations", type=int, default=None)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default Jacobi plan")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
-    args = ap.parse_args()
+def latency(ms):
+    import numpy as np
+    return {"p50": round(float(statistics.median(ms)), 4),
+            "p99": round(float(np.percentile(ms, 99)), 4)}
 
-    if args.gpus != 1:
-        print(f"bench.py: only single-GPU runs are supported (asked for --gpus {args.gpus}); "
-              "the step shards by z-slabs across GPUs only in design, not in this build",
-              file=sys.stderr)
-        sys.exit(2)
 
+def run_single(args, sc):
     import numpy as np
     import smokesim
-
-    log(smokesim.device_info())
-    sc = workload(args.workload)
     if args.projection:
         sc["params"]["projection"] = {"reference": 0, "rbsor": 1, "jacobi": 2}[args.projection]
     if args.iterations is not None:
@@ -161,19 +141,18 @@ def main():
     log(f"timed: {args.steps} steps in {wall:.3f} s; density sum {float(dens.sum()):.3f}")
 
     ms_per_step = 1000.0 * wall / args.steps
-    value = cells * args.steps / wall
     p = sc["params"]
-    bpc = sweep_bytes_per_cell(p)
+    bpc = step_bytes_per_cell(p)
 
     # Dominant kernel: the pressure sweep.  Algorithmic bytes per launch =
-    # 13 B x interior cells (the sweep's domain); duration from HIP events
-    # around the sweep loops on the simulation's stream.
+    # 13 B x interior cells (the sweep's domain) x sweeps per launch;
+    # duration from HIP events around the sweep loops on the simulation's
+    # stream.
     interior = (nx - 2) * (ny - 2) * (nz - 2)
     roofline = None
     if sweep_launches > 0:
         avg_ms = sweep_ms / sweep_launches
         if p["projection"] == 2:
-            # fused launches: sweeps per launch = all timed sweeps / launches
             per_launch = p["iterations"] * timing["steps"] / sweep_launches
             alg = BYTES_SWEEP * interior * per_launch
             kernel = "k_jacobi_fused"
@@ -202,8 +181,8 @@ def main():
 
     step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
     out = {
-        "metric": "grid cell-updates/s per full step",
-        "value": round(value, 1),
+        "metric": METRIC,
+        "value": round(cells * args.steps / wall, 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -218,10 +197,7 @@ def main():
             "workload": describe(sc), "cells": cells, "projection": p["projection"],
             "iterations": p["iterations"], "parallelism": "single-gpu",
         },
-        "latency_ms": {
-            "p50": round(float(statistics.median(step_ms)), 4),
-            "p99": round(float(np.percentile(step_ms, 99)), 4),
-        },
+        "latency_ms": latency(step_ms),
         "stage_ms_per_step": {k: round(v / max(timing["steps"], 1), 4)
                               for k, v in timing["stage_ms"].items()},
         "step_roofline": {"bytes_per_cell": bpc, "achieved": round(step_eff, 1),
@@ -234,6 +210,97 @@ def main():
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 50)
     sim.close()
+    return out
+
+
+def run_batch(args, scs):
+    import numpy as np
+    import smokesim
+    if args.iterations is not None:
+        for sc in scs:
+            sc["params"]["iterations"] = args.iterations
+    sims = []
+    for sc in scs:
+        s = smokesim.Simulation(*sc["dims"])
+        smokesim.scenes.apply(s, sc)
+        sims.append(s)
+    b = smokesim.Batch(sims, [sc["dt"] for sc in scs])
+    cells = b.cells
+    k = scs[0]["params"]["iterations"]
+    log(f"batch: {len(scs)} simulations, {cells} cells, Jacobi x{k}")
+    tuned = None
+    if not args.no_autotune:
+        plan, ms = b.autotune()
+        tuned = {"plan": plan, "ms_per_sweep": round(ms, 5)}
+        log(f"batch autotune: {plan}")
+    for _ in range(args.warmup):
+        b.step(use_graph=True)
+    b.synchronize()
+    n_warm = len(b.step_ms())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.step(use_graph=True)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = b.step_ms()[n_warm:]
+    dens = sims[0].read_density()
+    if not np.isfinite(dens).all():
+        raise SystemExit("non-finite density after the timed steps")
+    ms_per_step = 1000.0 * wall / args.steps
+    bpc = step_bytes_per_cell(scs[0]["params"])
+    step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(cells * args.steps / wall, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch_64: {len(scs)} simulations, dims uniform in [32,160]^3, "
+                               f"0-10% boxes, 1-3 sources, dt in [0.02,0.08], Jacobi x{k}, "
+                               "one launch per stage, HIP graph",
+                   "cells": cells, "simulations": len(scs), "parallelism": "single-gpu"},
+        "latency_ms": latency(step_ms),
+        "step_roofline": {"bytes_per_cell": bpc, "achieved": round(step_eff, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_eff / HBM_PEAK_GBS, 4)},
+        "roofline": None, "autotune": tuned, "cpu_baseline": None,
+    }
+    b.close()
+    for s in sims:
+        s.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="grid_256",
+                    choices=["grid_256", "grid_512", "reference_80", "batch_64"])
+    ap.add_argument("--projection", choices=["jacobi", "rbsor", "reference"], default=None)
+    ap.add_argument("--iterations", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default Jacobi plan")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: only single-GPU runs are supported (asked for --gpus {args.gpus}); "
+              "the step shards by z-slabs across GPUs only in design, not in this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import smokesim
+    from smokesim import scenes
+    log(smokesim.device_info())
+    if args.workload == "batch_64":
+        out = run_batch(args, scenes.batch_64())
+    else:
+        sc = {"grid_256": scenes.grid_256, "grid_512": scenes.grid_512,
+              "reference_80": scenes.reference_80}[args.workload]()
+        out = run_single(args, sc)
     print(json.dumps(out), flush=True)
 
 

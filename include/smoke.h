@@ -249,6 +249,39 @@ smoke_status smoke_timing_sweeps(smoke_sim* sim, double* ms, long long* launches
  * receives how many were written. */
 smoke_status smoke_timing_step_ms(smoke_sim* sim, double* ms, int cap, int* count);
 
+/* --------------------------------------------------------------- batches */
+/* Many independent simulations of different sizes advanced together: one
+ * kernel launch per stage for all of them (persistent kernels over a
+ * (simulation, tile) work list), optionally replayed from a captured HIP
+ * graph.  No reference counterpart (the reference runs one global
+ * simulation).  Every simulation gives the same bits in a batch as alone.
+ *
+ * Requirements: at most SMOKE_MAX_BATCH simulations on the current device,
+ * all with the Jacobi projection, cold start, and equal iteration count and
+ * jacobi_omega; at most 8 sources each.  While a simulation belongs to a
+ * batch its scene is frozen and it is stepped only through the batch (the
+ * per-simulation step, scene and parameter calls return SMOKE_ERR_STATE);
+ * field reads wait for the batch.  dts: one time step per simulation. */
+#define SMOKE_MAX_BATCH 256
+typedef struct smoke_batch smoke_batch;
+smoke_status smoke_batch_create(smoke_sim* const* sims, int count, const float* dts,
+                                smoke_batch** out);
+void smoke_batch_destroy(smoke_batch* batch);
+/* Enqueues one step of every simulation; use_graph != 0 captures the step
+ * into a HIP graph on first use and replays it afterwards. */
+smoke_status smoke_batch_step(smoke_batch* batch, int use_graph);
+smoke_status smoke_batch_synchronize(smoke_batch* batch);
+smoke_status smoke_batch_set_jacobi_plan(smoke_batch* batch, const smoke_jacobi_plan* plan);
+smoke_status smoke_batch_get_jacobi_plan(const smoke_batch* batch, smoke_jacobi_plan* out);
+/* Autotuner of the batched sweep sequence (same candidates as
+ * smoke_autotune); invalidates a captured graph. */
+smoke_status smoke_batch_autotune(smoke_batch* batch, smoke_jacobi_plan* chosen, double* ms_per_sweep);
+/* Step latencies (ms, HIP events around each step on the batch's stream) of
+ * the last `cap` steps, oldest first. */
+smoke_status smoke_batch_step_ms(smoke_batch* batch, double* ms, int cap, int* count);
+/* Cells of all simulations of the batch (for cell-updates/s). */
+long long smoke_batch_cells(const smoke_batch* batch);
+
 /* ------------------------------------------------------ legacy interface */
 /* Process-global single simulation behind the reference's nine functions
  * (smokeSimulation.cuh:4-17).  Used by smokeSimulation.cuh; errors are

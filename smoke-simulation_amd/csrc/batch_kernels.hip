@@ -1,0 +1,260 @@
+// Batched smoke step: many simulations of different sizes advance together
+// with one launch per stage.  Each launch is a persistent grid that walks a
+// flattened work list of (simulation, block) "virtual blocks"; the arithmetic
+// is the single-simulation arithmetic of step_device.h, so every simulation
+// gives the same bits in a batch as alone.
+//
+// Virtual blocks are handed to XCDs in contiguous runs: workgroup b runs on
+// XCD b % 8, so the grid's workgroups of XCD j take the j-th eighth of the
+// work list, where neighbouring blocks of one simulation (sharing rows and
+// halo columns) meet in the same L2.
+#include "step_device.h"
+
+namespace smk {
+
+namespace {
+
+using dev::BX;
+using dev::BY;
+
+// Virtual blocks of this workgroup: the XCD's contiguous share of [0, total)
+// walked with a stride of the XCD's workgroup count.  gridDim.x is a multiple
+// of 8 (launchers guarantee it).
+struct VbRange {
+    int begin, end, stride;
+};
+__device__ __forceinline__ VbRange vb_range(int total)
+{
+    const int xcd = (int)blockIdx.x & 7;
+    const int per_xcd = (int)gridDim.x >> 3;
+    const int chunk = (total + 7) >> 3;
+    VbRange r;
+    r.begin = xcd * chunk + ((int)blockIdx.x >> 3);
+    r.end = min((xcd + 1) * chunk, total);
+    r.stride = per_xcd;
+    return r;
+}
+
+// Simulation owning virtual block vb: last s with start[s] <= vb (binary
+// search over the prefix table; uniform across the workgroup).
+__device__ __forceinline__ int find_owner(const int* __restrict__ start, int n, int vb)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= vb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(BX * BY) void kb_sources(const SimDesc* __restrict__ sims,
+                                                      const SourceTable* __restrict__ tab)
+{
+    const int total = tab->start[tab->count];
+    const VbRange r = vb_range(total);
+    for (int vb = r.begin; vb < r.end; vb += r.stride) {
+        const int k = find_owner(tab->start, tab->count, vb);
+        const int lb = vb - tab->start[k];
+        const int4 lo = tab->lo[k], ext = tab->ext[k];
+        const int gy = (ext.y + BY - 1) / BY;
+        const int bx = lb % ext.w, by = (lb / ext.w) % gy, bz = lb / (ext.w * gy);
+        const int x = lo.x + bx * BX + (int)threadIdx.x;
+        const int y = lo.y + by * BY + (int)threadIdx.y;
+        const int z = lo.z + bz;
+        if (x >= lo.x + ext.x || y >= lo.y + ext.y) continue;
+        const SimDesc& s = sims[tab->sim[k]];
+        dev::source_at(s.g, s.rho[0], s.rho[1], tab->sph[k], x, y, z);
+    }
+}
+
+// Per-cell stages.  The coordinate offsets match the single-simulation
+// launches: force covers y from 1, the others start at 1 on every axis.
+template <int STAGE>
+__global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ sims,
+                                                    const StageTable* __restrict__ tab, int nsims,
+                                                    int lp)
+{
+    const int total = tab->start[nsims];
+    const VbRange r = vb_range(total);
+    for (int vb = r.begin; vb < r.end; vb += r.stride) {
+        const int si = find_owner(tab->start, nsims, vb);
+        const int lb = vb - tab->start[si];
+        const int gx = tab->gx[si], gy = tab->gy[si];
+        const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+        const SimDesc& s = sims[si];
+        const int n = s.now, p = 1 - s.now;
+        const int x = bx * BX + (int)threadIdx.x + (STAGE == BS_FORCE ? 0 : 1);
+        const int y = by * BY + (int)threadIdx.y + 1;
+        const int z = bz + (STAGE == BS_FORCE ? 0 : 1);
+        if (STAGE == BS_FORCE)
+            dev::force_at(s.g, s.u[n], s.v[n], s.w[n], s.rho[n], s.flags, s.force, x, y, z);
+        else if (STAGE == BS_DIV)
+            dev::divergence_at(s.g, s.u[n], s.v[n], s.w[n], s.flags, s.d, x, y, z);
+        else if (STAGE == BS_GRAD)
+            dev::gradient_at(s.g, s.u[n], s.v[n], s.w[n], s.P[s.pcur ^ lp], s.flags, x, y, z);
+        else if (STAGE == BS_ADV_V)
+            dev::advect_velocity_at(s.g, s.u[n], s.v[n], s.w[n], s.u[p], s.v[p], s.w[p], s.flags,
+                                    s.force.dt, x, y, z);
+        else
+            dev::advect_density_at(s.g, s.rho[n], s.rho[p], s.u[p], s.v[p], s.w[p], s.flags,
+                                   s.force.dt, x, y, z);
+    }
+}
+
+// Fused Jacobi sweeps over every tile of every simulation.
+template <int TX, int TY, int T, bool DAMPED>
+__global__ __launch_bounds__(TX * TY) void kb_jacobi(const SimDesc* __restrict__ sims,
+                                                     const StageTable* __restrict__ tab, int nsims,
+                                                     int lp, int zc, int zero_in, float omega)
+{
+    __shared__ dev::FusedLds<TX, TY, T> sh;
+    const int total = tab->start[nsims];
+    const VbRange r = vb_range(total);
+    for (int vb = r.begin; vb < r.end; vb += r.stride) {
+        const int si = find_owner(tab->start, nsims, vb);
+        const int lb = vb - tab->start[si];
+        const int gx = tab->gx[si], gy = tab->gy[si];
+        const SimDesc& s = sims[si];
+        const int in = s.pcur ^ lp;
+        dev::jacobi_fused_tile<TX, TY, T, DAMPED>(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, zc,
+                                                  zero_in, omega, lb % gx, (lb / gx) % gy,
+                                                  lb / (gx * gy), sh);
+    }
+}
+
+// Single sweeps (the remainder of K when it is 1), one cell per thread.
+__global__ __launch_bounds__(BX * BY) void kb_jacobi1(const SimDesc* __restrict__ sims,
+                                                      const StageTable* __restrict__ tab,
+                                                      int nsims, int lp, int zero_in, float omega,
+                                                      int plain)
+{
+    const int total = tab->start[nsims];
+    const VbRange r = vb_range(total);
+    for (int vb = r.begin; vb < r.end; vb += r.stride) {
+        const int si = find_owner(tab->start, nsims, vb);
+        const int lb = vb - tab->start[si];
+        const int gx = tab->gx[si], gy = tab->gy[si];
+        const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+        const SimDesc& s = sims[si];
+        const int in = s.pcur ^ lp;
+        dev::jacobi_at(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, omega, plain, zero_in,
+                       bx * BX + (int)threadIdx.x + 1, by * BY + (int)threadIdx.y + 1, bz + 1);
+    }
+}
+
+__global__ void kb_flip(SimDesc* __restrict__ sims, int nsims, int pflip)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= nsims) return;
+    sims[i].now ^= 1;
+    sims[i].pcur ^= pflip;
+}
+
+template <int TX, int TY, int T>
+hipError_t launch_bj(const SimDesc* sims, const StageTable* tab, int nsims, int grid, int lp,
+                     int zc, int zero_in, float omega, int plain, hipStream_t st)
+{
+    if (plain)
+        hipLaunchKernelGGL((kb_jacobi<TX, TY, T, false>), dim3((unsigned)grid), dim3(TX, TY), 0, st,
+                           sims, tab, nsims, lp, zc, zero_in, omega);
+    else
+        hipLaunchKernelGGL((kb_jacobi<TX, TY, T, true>), dim3((unsigned)grid), dim3(TX, TY), 0, st,
+                           sims, tab, nsims, lp, zc, zero_in, omega);
+    return hipGetLastError();
+}
+
+template <int TX, int TY>
+hipError_t launch_bj_t(int t, const SimDesc* sims, const StageTable* tab, int nsims, int grid,
+                       int lp, int zc, int zero_in, float omega, int plain, hipStream_t st)
+{
+    switch (t) {
+    case 2: return launch_bj<TX, TY, 2>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 3: return launch_bj<TX, TY, 3>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 4: return launch_bj<TX, TY, 4>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 5: return launch_bj<TX, TY, 5>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 6: return launch_bj<TX, TY, 6>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename K>
+int occupancy(K kernel, int threads)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1)
+        n = 1;
+    return n;
+}
+
+}  // namespace
+
+hipError_t batch_sources(const SimDesc* sims, const SourceTable* tab, int total_blocks, int grid,
+                         hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_sources, dim3((unsigned)grid), dim3(BX, BY), 0, st, sims, tab);
+    return hipGetLastError();
+}
+
+hipError_t batch_cells(int stage, const SimDesc* sims, const StageTable* tab, int nsims,
+                       int total_blocks, int grid, int lp, hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    const dim3 gd((unsigned)grid), bd(BX, BY);
+    switch (stage) {
+    case BS_FORCE: hipLaunchKernelGGL(kb_cells<BS_FORCE>, gd, bd, 0, st, sims, tab, nsims, lp); break;
+    case BS_DIV: hipLaunchKernelGGL(kb_cells<BS_DIV>, gd, bd, 0, st, sims, tab, nsims, lp); break;
+    case BS_GRAD: hipLaunchKernelGGL(kb_cells<BS_GRAD>, gd, bd, 0, st, sims, tab, nsims, lp); break;
+    case BS_ADV_V: hipLaunchKernelGGL(kb_cells<BS_ADV_V>, gd, bd, 0, st, sims, tab, nsims, lp); break;
+    case BS_ADV_D: hipLaunchKernelGGL(kb_cells<BS_ADV_D>, gd, bd, 0, st, sims, tab, nsims, lp); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_jacobi(const SimDesc* sims, const StageTable* tab, int nsims, int total_blocks,
+                        int grid, const JacobiPlan& plan, int sweeps, int lp, int zero_in,
+                        float omega, int plain, hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    if (sweeps == 1) {
+        hipLaunchKernelGGL(kb_jacobi1, dim3((unsigned)grid), dim3(BX, BY), 0, st, sims, tab, nsims,
+                           lp, zero_in, omega, plain);
+        return hipGetLastError();
+    }
+    if (plan.zc < 1) return hipErrorInvalidValue;
+    switch (plan.tile) {
+    case JacobiPlan::TILE_32x32:
+        return launch_bj_t<32, 32>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
+                                   plain, st);
+    case JacobiPlan::TILE_64x16:
+        return launch_bj_t<64, 16>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
+                                   plain, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t batch_flip(SimDesc* sims, int nsims, int pflip, hipStream_t st)
+{
+    hipLaunchKernelGGL(kb_flip, dim3((unsigned)((nsims + 63) / 64)), dim3(64), 0, st, sims, nsims,
+                       pflip);
+    return hipGetLastError();
+}
+
+int batch_cells_blocks_per_cu() { return occupancy(kb_cells<BS_ADV_V>, BX * BY); }
+
+int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps)
+{
+    if (sweeps == 1) return occupancy(kb_jacobi1, BX * BY);
+    const bool big = plan.tile == JacobiPlan::TILE_32x32;
+    switch (sweeps) {
+    case 2: return big ? occupancy(kb_jacobi<32, 32, 2, false>, 1024) : occupancy(kb_jacobi<64, 16, 2, false>, 1024);
+    case 3: return big ? occupancy(kb_jacobi<32, 32, 3, false>, 1024) : occupancy(kb_jacobi<64, 16, 3, false>, 1024);
+    case 4: return big ? occupancy(kb_jacobi<32, 32, 4, false>, 1024) : occupancy(kb_jacobi<64, 16, 4, false>, 1024);
+    case 5: return big ? occupancy(kb_jacobi<32, 32, 5, false>, 1024) : occupancy(kb_jacobi<64, 16, 5, false>, 1024);
+    default: return big ? occupancy(kb_jacobi<32, 32, 6, false>, 1024) : occupancy(kb_jacobi<64, 16, 6, false>, 1024);
+    }
+}
+
+}  // namespace smk

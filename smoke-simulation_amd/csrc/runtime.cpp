@@ -17,15 +17,17 @@
 #include <string>
 #include <vector>
 
-#include "smoke_internal.h"
+#include "sim_state.h"
 
 using smk::Grid;
+using smk::fail;
+using smk::Object;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-smoke_status fail(smoke_status st, const char* fmt, ...)
+smoke_status smk::fail(smoke_status st, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -36,19 +38,10 @@ smoke_status fail(smoke_status st, const char* fmt, ...)
     return st;
 }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(SMOKE_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
-    } while (0)
+namespace {
 
-struct Object {
-    float pos[3];
-    float vel[3];
-    float r;
-    int type;  // 0 obstacle, 1 source (smokeSimulation.cu:50)
-};
+
+
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
@@ -103,64 +96,6 @@ smoke_status check_params(const smoke_params* p)
 
 }  // namespace
 
-struct smoke_sim {
-    std::mutex mu;
-    Grid g{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    char* arena = nullptr;
-    float* rho[2] = {nullptr, nullptr};
-    float* u[2] = {nullptr, nullptr};
-    float* v[2] = {nullptr, nullptr};
-    float* w[2] = {nullptr, nullptr};
-    float* P[2] = {nullptr, nullptr};
-    float* d = nullptr;
-    float* staging = nullptr;
-    uint8_t* base = nullptr;
-    uint8_t* s = nullptr;
-    uint8_t* flags = nullptr;
-
-    smoke_params prm{};
-    std::vector<Object> objects;
-    int now = 1;   // indexNow (smokeSimulation.cu:707)
-    int pcur = 0;  // P buffer holding the latest pressure
-    bool mask_dirty = true;
-    smk::JacobiPlan plan;  // launch plan of the fused Jacobi sweeps
-    long long steps_enqueued = 0;
-
-    // Asynchronous read-back: a copy stream, two device pack slots and two
-    // pinned host buffers (allocated on first use).
-    struct Readback {
-        int slot;
-        long long step;
-    };
-    hipStream_t copy_stream = nullptr;
-    void* rb_dev = nullptr;          // 2 slots of nx*ny*nz*4 bytes
-    void* rb_host[2] = {nullptr, nullptr};
-    hipEvent_t rb_step_done = nullptr, rb_packed[2] = {nullptr, nullptr},
-               rb_copied[2] = {nullptr, nullptr};
-    std::deque<Readback> rb_pending;
-    int rb_next = 0;
-
-    // Timing: events at every stage boundary of a step, plus two around the
-    // projection's sweep launches (the step's dominant kernel).
-    bool timing = false;
-    struct StepEvents {
-        hipEvent_t ev[SMOKE_STAGE_COUNT + 1];
-        long long launches[SMOKE_STAGE_COUNT];
-        hipEvent_t sweep_ev[2];
-        long long sweeps;
-    };
-    StepEvents* cur = nullptr;  // events of the step being enqueued
-    std::vector<StepEvents> pending;
-    std::vector<hipEvent_t> event_pool;
-    double stage_ms[SMOKE_STAGE_COUNT] = {0};
-    long long stage_launches[SMOKE_STAGE_COUNT] = {0};
-    double sweep_ms = 0.0;
-    long long sweep_launches = 0;
-    long long timed_steps = 0;
-    std::deque<double> step_ms;
-};
 
 namespace {
 
@@ -185,7 +120,10 @@ int count_type(const smoke_sim* s, int type)
 // Rebuilds mask and flags from the base mask and the obstacles.  The mask is a
 // pure function of those two, so it is recomputed only when one changed; the
 // reference recomputes it every step (drawObjects, smokeSimulation.cu:766).
-hipError_t refresh_mask(smoke_sim* s, long long* launches)
+
+}  // namespace
+
+hipError_t smk::refresh_mask(smoke_sim* s, long long* launches)
 {
     if (!s->mask_dirty) return hipSuccess;
     smk::SphereList obst = sphere_list(s, 0);
@@ -197,6 +135,10 @@ hipError_t refresh_mask(smoke_sim* s, long long* launches)
     if (launches) *launches += 2;
     return hipSuccess;
 }
+
+namespace {
+
+using smk::refresh_mask;
 
 hipError_t stage_scene(smoke_sim* s, long long* launches)
 {
@@ -389,6 +331,7 @@ hipError_t collect_timing(smoke_sim* s, bool wait)
 
 smoke_status step_locked(smoke_sim* s, float dt)
 {
+    SMOKE_REFUSE_IN_BATCH(s, "step");
     if (!std::isfinite(dt)) return fail(SMOKE_ERR_INVALID, "step: dt must be finite");
     smoke_sim::StepEvents se{};
     if (s->timing) {
@@ -421,6 +364,7 @@ smoke_status step_locked(smoke_sim* s, float dt)
 // buffer.
 smoke_status read_cells_f32(smoke_sim* s, const float* src, float* dst)
 {
+    SMOKE_WAIT_BATCH(s);
     HIP_TRY(smk::pack_cells_f32(s->g, src, s->staging, s->stream));
     size_t bytes = (size_t)s->g.nx * s->g.ny * s->g.nz * sizeof(float);
     HIP_TRY(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, s->stream));
@@ -537,6 +481,7 @@ smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* de
 void smoke_destroy(smoke_sim* s)
 {
     if (!s) return;
+    if (s->batch) smk::batch_detach(s->batch, s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& se : s->pending) {
         for (int k = 0; k <= SMOKE_STAGE_COUNT; ++k) (void)hipEventDestroy(se.ev[k]);
@@ -592,8 +537,10 @@ smoke_status smoke_readback_begin(smoke_sim* s, int format)
     const float* src = s->rho[1 - s->now];
     // The copy stream packs after the last enqueued step; the next step's
     // scene stage (which writes sources into both density halves) waits for
-    // the pack, not for the host copy.
-    HIP_TRY(hipEventRecord(s->rb_step_done, s->stream));
+    // the pack, not for the host copy.  A simulation in a batch is stepped on
+    // the batch's stream.
+    hipStream_t compute = s->batch_stream ? s->batch_stream : s->stream;
+    HIP_TRY(hipEventRecord(s->rb_step_done, compute));
     HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->rb_step_done, 0));
     size_t bytes = slot_bytes;
     if (format == SMOKE_READBACK_F32) {
@@ -603,7 +550,7 @@ smoke_status smoke_readback_begin(smoke_sim* s, int format)
         bytes = cells * sizeof(uint16_t);
     }
     HIP_TRY(hipEventRecord(s->rb_packed[slot], s->copy_stream));
-    HIP_TRY(hipStreamWaitEvent(s->stream, s->rb_packed[slot], 0));
+    HIP_TRY(hipStreamWaitEvent(compute, s->rb_packed[slot], 0));
     HIP_TRY(hipMemcpyAsync(s->rb_host[slot], dev, bytes, hipMemcpyDeviceToHost, s->copy_stream));
     HIP_TRY(hipEventRecord(s->rb_copied[slot], s->copy_stream));
     s->rb_pending.push_back({slot, s->steps_enqueued});
@@ -639,6 +586,7 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
     smoke_status st = check_params(p);
     if (st != SMOKE_OK) return st;
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "set_params");
     if (p->obstacle_mode != s->prm.obstacle_mode) s->mask_dirty = true;
     s->prm = *p;
     return SMOKE_OK;
@@ -673,6 +621,7 @@ smoke_status smoke_autotune(smoke_sim* s, smoke_jacobi_plan* chosen, double* ms_
 {
     if (!s) return fail(SMOKE_ERR_INVALID, "autotune: NULL sim");
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "autotune");
     const int K = s->prm.iterations > 0 ? s->prm.iterations : 100;
     HIP_TRY(refresh_mask(s, nullptr));
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -741,6 +690,7 @@ static smoke_status add_object(smoke_sim* s, int type, float x, float y, float z
 {
     if (!s) return fail(SMOKE_ERR_INVALID, "add object: NULL sim");
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "add object");
     if (count_type(s, type) >= SMOKE_MAX_OBJECTS)
         return fail(SMOKE_ERR_INVALID, "add object: more than %d %s", SMOKE_MAX_OBJECTS,
                     type == 0 ? "obstacles" : "sources");
@@ -770,6 +720,7 @@ smoke_status smoke_move_object(smoke_sim* s, int id, float x, float y, float z)
 {
     if (!s) return fail(SMOKE_ERR_INVALID, "move_object: NULL sim");
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "move_object");
     if (id < 0 || id >= (int)s->objects.size())
         return fail(SMOKE_ERR_INVALID, "move_object: no object with id %d", id);
     Object& o = s->objects[id];
@@ -782,6 +733,7 @@ smoke_status smoke_set_solids(smoke_sim* s, const uint8_t* solid)
 {
     if (!s || !solid) return fail(SMOKE_ERR_INVALID, "set_solids: NULL");
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "set_solids");
     size_t n = (size_t)s->g.nx * s->g.ny * s->g.nz;
     HIP_TRY(hipMemcpyAsync(s->staging, solid, n, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(smk::unpack_cells_solid(s->g, reinterpret_cast<const uint8_t*>(s->staging), s->base, s->stream));
@@ -795,6 +747,7 @@ smoke_status smoke_set_density(smoke_sim* s, int half, const float* src)
 {
     if (!s || !src || (half != 0 && half != 1)) return fail(SMOKE_ERR_INVALID, "set_density: bad argument");
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "set_density");
     size_t bytes = (size_t)s->g.nx * s->g.ny * s->g.nz * sizeof(float);
     HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(smk::unpack_cells_f32(s->g, s->staging, s->rho[half], s->stream));
@@ -826,6 +779,7 @@ smoke_status smoke_set_velocity(smoke_sim* s, int comp, int half, const float* s
     float* dst = comp_ptr(s, comp, half);
     if (!dst) return fail(SMOKE_ERR_INVALID, "set_velocity: bad component %d or half %d", comp, half);
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "set_velocity");
     size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
     HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(smk::unpack_faces_f32(s->g, s->staging, dst, s->stream));
@@ -839,6 +793,7 @@ smoke_status smoke_get_velocity(smoke_sim* s, int comp, int half, float* dst)
     float* src = comp_ptr(s, comp, half);
     if (!src) return fail(SMOKE_ERR_INVALID, "get_velocity: bad component %d or half %d", comp, half);
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_WAIT_BATCH(s);
     HIP_TRY(smk::pack_faces_f32(s->g, src, s->staging, s->stream));
     size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
     HIP_TRY(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, s->stream));
@@ -848,6 +803,7 @@ smoke_status smoke_get_velocity(smoke_sim* s, int comp, int half, float* dst)
 
 static smoke_status read_bytes(smoke_sim* s, const uint8_t* src, uint8_t* dst)
 {
+    SMOKE_WAIT_BATCH(s);
     uint8_t* st = reinterpret_cast<uint8_t*>(s->staging);
     HIP_TRY(smk::pack_cells_u8(s->g, src, st, s->stream));
     HIP_TRY(hipMemcpyAsync(dst, st, (size_t)s->g.nx * s->g.ny * s->g.nz, hipMemcpyDeviceToHost, s->stream));
@@ -901,6 +857,7 @@ smoke_status smoke_run_stage(smoke_sim* s, int stage, float dt)
     if (stage < 0 || stage >= SMOKE_STAGE_COUNT) return fail(SMOKE_ERR_INVALID, "run_stage: bad stage %d", stage);
     if (!std::isfinite(dt)) return fail(SMOKE_ERR_INVALID, "run_stage: dt must be finite");
     std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "run_stage");
     long long launches = 0;
     hipError_t e = run_stage(s, stage, dt, &launches);
     if (e != hipSuccess) return fail(SMOKE_ERR_HIP, "run_stage %d: %s", stage, hipGetErrorString(e));
@@ -917,6 +874,7 @@ smoke_status smoke_read_density(smoke_sim* s, float* dst)
 smoke_status smoke_synchronize(smoke_sim* s)
 {
     if (!s) return fail(SMOKE_ERR_INVALID, "synchronize: NULL sim");
+    SMOKE_WAIT_BATCH(s);
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SMOKE_OK;
 }

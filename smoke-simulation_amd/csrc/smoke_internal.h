@@ -28,6 +28,10 @@ struct Grid {
     }
 };
 
+// Thread block of the per-cell kernels (64 x 4: one wave64 per row segment).
+constexpr int kBlockX = 64;
+constexpr int kBlockY = 4;
+
 // Cell flag bits (one byte per cell).
 enum : uint8_t {
     F_SELF = 1,
@@ -62,6 +66,63 @@ struct JacobiPlan {
     int zc = 64;
 };
 
+// ------------------------------------------------------------- batching
+
+// One simulation as the batched kernels see it (device memory).  `now` and
+// `pcur` are flipped on the device at the end of every batched step, so a
+// captured step replays correctly.
+struct SimDesc {
+    Grid g;
+    float* rho[2];
+    float* u[2];
+    float* v[2];
+    float* w[2];
+    float* P[2];
+    float* d;
+    const uint8_t* flags;
+    ForceArgs force;  // includes this simulation's dt
+    int now;          // ping-pong parity (indexNow)
+    int pcur;         // P buffer holding the latest pressure
+};
+
+// Work list of one batched stage: simulation s owns virtual blocks
+// [start[s], start[s+1]) arranged as a gx[s] x gy[s] x (rest) block grid.
+constexpr int kMaxBatch = 256;
+struct StageTable {
+    int start[kMaxBatch + 1];
+    int gx[kMaxBatch];
+    int gy[kMaxBatch];
+};
+
+// Source fills of a batch: item k fills one source sphere of simulation
+// sim[k] over the box lo[k] + [0, ext[k]) (clipped to the interior).
+constexpr int kMaxBatchSources = kMaxBatch * 8;
+struct SourceTable {
+    int count;
+    int start[kMaxBatchSources + 1];  // virtual blocks per item (prefix)
+    int sim[kMaxBatchSources];
+    float4 sph[kMaxBatchSources];
+    int4 lo[kMaxBatchSources];   // x, y, z of the box corner
+    int4 ext[kMaxBatchSources];  // ex, ey, ez, blocks in x
+};
+
+enum BatchStage { BS_FORCE = 0, BS_DIV = 1, BS_GRAD = 2, BS_ADV_V = 3, BS_ADV_D = 4 };
+
+// Batched launchers (batch_kernels.hip).  `grid` is the persistent grid size
+// (a multiple of 8); `lp` is the parity of this launch within the step's
+// sweep sequence (input P = P[pcur ^ lp]).
+hipError_t batch_sources(const SimDesc* sims, const SourceTable* tab, int total_blocks, int grid,
+                         hipStream_t st);
+hipError_t batch_cells(int stage, const SimDesc* sims, const StageTable* tab, int nsims,
+                       int total_blocks, int grid, int lp, hipStream_t st);
+hipError_t batch_jacobi(const SimDesc* sims, const StageTable* tab, int nsims, int total_blocks,
+                        int grid, const JacobiPlan& plan, int sweeps, int lp, int zero_in,
+                        float omega, int plain, hipStream_t st);
+hipError_t batch_flip(SimDesc* sims, int nsims, int pflip, hipStream_t st);
+// Resident blocks per CU of the batched kernels (for sizing persistent grids).
+int batch_cells_blocks_per_cu();
+int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps);
+
 // ------------------------------------------------------------ launchers
 // Every launcher enqueues on `st` and returns hipGetLastError().
 
@@ -86,6 +147,9 @@ hipError_t build_mask(const Grid& g, const uint8_t* base, uint8_t* s, const Sphe
 hipError_t build_flags(const Grid& g, const uint8_t* s, uint8_t* flags, hipStream_t st);
 hipError_t fill_source(const Grid& g, float* rho0, float* rho1, float4 sph, hipStream_t st,
                        int* launched);
+// Conservative bounding box of a source sphere clipped to the interior:
+// false if it holds no cell.
+bool source_box(const Grid& g, float4 sph, int lo[3], int ext[3]);
 
 // Force: buoyancy + clamp on [now].
 hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,

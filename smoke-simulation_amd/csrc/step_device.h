@@ -1,0 +1,468 @@
+// Device-side arithmetic of one smoke step, shared by the single-simulation
+// kernels (step_kernels.hip) and the batched persistent kernels
+// (batch_kernels.hip), so a simulation gives the same bits alone and in a
+// batch.
+//
+// Every function evaluates each output with the same operations, in the same
+// order, as the CPU oracle (oracle/smoke_oracle.cpp), which restates
+// project/smokeSimulation.cu.  Files including this header are compiled with
+// -ffp-contract=off (no multiply-add is fused behind the source's back) and
+// HIP's correctly rounded FP32 division, so the results are bit-identical to
+// the oracle's FP32 mode.
+//
+// The *_at functions take the cell/face coordinates a thread owns and check
+// their own range; the caller maps threads to coordinates.
+#pragma once
+
+#include "smoke_internal.h"
+
+namespace smk {
+namespace dev {
+
+// Thread block of the per-cell kernels: one wave64 covers 64 consecutive x of
+// one row (one contiguous 256-byte run of a 128-byte-aligned FP32 row).
+constexpr int BX = kBlockX;
+constexpr int BY = kBlockY;
+
+// Sphere membership as fillSmoke/fillObstacle test it
+// (smokeSimulation.cu:265, :304): integer cell index, float centre, squares
+// and sums in float, strict comparison with r*r.
+__device__ __forceinline__ bool inside(float4 s, int x, int y, int z)
+{
+    float dx = (float)x - s.x;
+    float dy = (float)y - s.y;
+    float dz = (float)z - s.z;
+    float dist = dx * dx + dy * dy + dz * dz;
+    return dist < s.w * s.w;
+}
+
+__device__ __forceinline__ bool is_interior(const Grid& g, int x, int y, int z)
+{
+    return x >= 1 && x <= g.nx - 2 && y >= 1 && y <= g.ny - 2 && z >= 1 && z <= g.nz - 2;
+}
+
+// fillSmoke (smokeSimulation.cu:251-273) for one cell the caller knows to be
+// interior: density 1.0 in both halves if the cell is inside the source.
+__device__ __forceinline__ void source_at(const Grid& g, float* __restrict__ r0,
+                                          float* __restrict__ r1, float4 sph, int x, int y, int z)
+{
+    if (inside(sph, x, y, z)) {
+        int i = g.idx(x, y, z);
+        r0[i] = 1.0f;
+        r1[i] = 1.0f;
+    }
+}
+
+// integrate (smokeSimulation.cu:315-329) and velocityConfinement (:331-352)
+// fused per staggered index: the buoyancy range is x in [0,nx-1],
+// y in [1,ny-1], z in [0,nz-1]; the clamp range x,y,z in [1,n-1] lies inside
+// it and sees the post-buoyancy v of the same index.
+__device__ __forceinline__ void force_at(const Grid& g, float* __restrict__ u,
+                                         float* __restrict__ v, float* __restrict__ w,
+                                         const float* __restrict__ rho,
+                                         const uint8_t* __restrict__ flags, const ForceArgs& a,
+                                         int x, int y, int z)
+{
+    if (x >= g.nx || y < 1 || y >= g.ny || z >= g.nz) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    float vv = v[i];
+    bool write_v = false;
+    if ((fl & F_SELF) && (fl & F_YM)) {
+        float r = rho[i];
+        if (!a.gate_on || r > a.gate) {
+            float b = a.buoyancy * (r - 0.0f);
+            vv = vv + (r * a.gravity * a.dt + b * a.dt);
+            write_v = true;
+        }
+    }
+    if (x >= 1 && z >= 1) {
+        float uu = u[i];
+        float ww = w[i];
+        float L = uu * uu + vv * vv + ww * ww;
+        if (L * a.dt > a.max_v2) {
+            u[i] = uu * (a.max_v2 / (L * a.dt));
+            vv = vv * (a.max_v2 / (L * a.dt));
+            w[i] = ww * (a.max_v2 / (L * a.dt));
+            write_v = true;
+        }
+    }
+    if (write_v) v[i] = vv;
+}
+
+// Colour-o cells of row (y,z): x = 2t + 1 + ((y+z+o+1) & 1) enumerates the
+// interior x with (x+y+z+o) even, as smokeSimulation.cu:362 does.
+__device__ __forceinline__ int colour_x(int t, int y, int z, int o)
+{
+    return 2 * t + 1 + ((y + z + o + 1) & 1);
+}
+
+// Right-hand side of the pressure form: d = div(u*) on active cells, in the
+// reference's operand order (smokeSimulation.cu:379-381); 0 elsewhere.
+__device__ __forceinline__ void divergence_at(const Grid& g, const float* __restrict__ u,
+                                              const float* __restrict__ v,
+                                              const float* __restrict__ w,
+                                              const uint8_t* __restrict__ flags,
+                                              float* __restrict__ d, int x, int y, int z)
+{
+    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2) return;
+    int i = g.idx(x, y, z);
+    float dv = 0.0f;
+    if (flags[i] & F_ACTIVE)
+        dv = -u[i] + u[i + 1] + -v[i] + v[i + g.px] + -w[i] + w[i + g.plane];
+    d[i] = dv;
+}
+
+__device__ __forceinline__ float nbr_sum(const float* __restrict__ p, int i, uint8_t fl, int px,
+                                         int plane)
+{
+    float sum = 0.0f;
+    if (fl & F_XM) sum += p[i - 1];
+    if (fl & F_XP) sum += p[i + 1];
+    if (fl & F_YM) sum += p[i - px];
+    if (fl & F_YP) sum += p[i + px];
+    if (fl & F_ZM) sum += p[i - plane];
+    if (fl & F_ZP) sum += p[i + plane];
+    return sum;
+}
+
+// RN(1/acc) for acc = 1..6: compile-time constants (correctly rounded by the
+// compiler) picked with selects, so no memory is touched.
+__device__ __forceinline__ float inv_acc(int acc)
+{
+    float r = 1.0f;
+    r = (acc == 2) ? 0.5f : r;
+    r = (acc == 3) ? (1.0f / 3.0f) : r;
+    r = (acc == 4) ? 0.25f : r;
+    r = (acc == 5) ? 0.2f : r;
+    r = (acc == 6) ? (1.0f / 6.0f) : r;
+    return r;
+}
+
+// Diagonal inverse of a cell, 0 for cells the projection does not relax.
+__device__ __forceinline__ float cell_inv(uint8_t fl)
+{
+    return (fl & F_ACTIVE) ? inv_acc(__popc(fl & F_NBRS)) : 0.0f;
+}
+
+// The pressure is +0 on every cell that is not active (ring, solid and
+// isolated cells are written 0 and never relaxed), and a running sum that
+// starts at +0 is never -0, so adding all six neighbours unmasked gives the
+// same bits as adding only the fluid ones.
+__device__ __forceinline__ float sum6(float a0, float a1, float a2, float a3, float a4, float a5)
+{
+    float sum = 0.0f;
+    sum = sum + a0;
+    sum = sum + a1;
+    sum = sum + a2;
+    sum = sum + a3;
+    sum = sum + a4;
+    sum = sum + a5;
+    return sum;
+}
+
+// One (damped) Jacobi sweep of acc_i P_i - sum_n s_n P_n = -d_i at an interior
+// cell; inactive cells are written 0.  zero_in: the input is all zeros (cold
+// start), so it is not read.
+__device__ __forceinline__ void jacobi_at(const Grid& g, const float* __restrict__ pin,
+                                          float* __restrict__ pout, const float* __restrict__ d,
+                                          const uint8_t* __restrict__ flags, float omega,
+                                          int plain, int zero_in, int x, int y, int z)
+{
+    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2) return;
+    int i = g.idx(x, y, z);
+    float inv = cell_inv(flags[i]);
+    float sum = 0.0f, pi = 0.0f;
+    if (!zero_in) {
+        sum = sum6(pin[i - 1], pin[i + 1], pin[i - g.px], pin[i + g.px], pin[i - g.plane],
+                   pin[i + g.plane]);
+        pi = pin[i];
+    }
+    float G = (sum - d[i]) * inv;
+    float out = plain ? G : pi + omega * (G - pi);
+    pout[i] = (inv > 0.0f) ? out : 0.0f;
+}
+
+// LDS of one fused-Jacobi block: per level, two planes of the grown tile,
+// padded by one on each side (edge threads read the never-written pad, which
+// only reaches halo columns that are not written out).
+template <int TX, int TY, int T>
+struct FusedLds {
+    float v[T][2][TY + 2][TX + 2];
+};
+
+// T fused Jacobi sweeps (temporal blocking) for one tile (tile_x, tile_y) and
+// z-chunk tile_z of a TX x TY thread block.  The block computes the tile
+// grown by T cells on each side in x and y and writes only the inner
+// (TX-2T) x (TY-2T) columns, whose values after T sweeps depend only on data
+// inside the grown tile.  Each thread holds one column and streams it in z:
+// level t (sweep t) at plane k-t is computed from level t-1 at planes
+// k-t-1, k-t, k-t+1 kept in registers and from the four x/y neighbours of
+// level t-1 at plane k-t held in LDS (double buffered, one barrier per plane).
+// Every value is computed with the single-sweep arithmetic, so results are
+// bit-identical to T single sweeps.  The next plane is prefetched into
+// registers one iteration ahead.  All threads of the block must call this
+// with the same tile (the barriers are block-wide).
+template <int TX, int TY, int T, bool DAMPED>
+__device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __restrict__ pin,
+                                                  float* __restrict__ pout,
+                                                  const float* __restrict__ d,
+                                                  const uint8_t* __restrict__ flags, int zc,
+                                                  int zero_in, float omega, int tile_x, int tile_y,
+                                                  int tile_z, FusedLds<TX, TY, T>& sh)
+{
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    constexpr int OX = TX - 2 * T, OY = TY - 2 * T;
+    const int x = tile_x * OX + tx - T + 1;
+    const int y = tile_y * OY + ty - T + 1;
+    const int z0 = tile_z * zc + 1;
+    const int z1 = min(z0 + zc, g.nz - 1);
+    const bool in_grid = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+    const bool writes = tx >= T && tx < TX - T && ty >= T && ty < TY - T && x <= g.nx - 2 &&
+                        y <= g.ny - 2;
+    const int base = in_grid ? g.idx(x, y, 0) : 0;
+
+    float w[T][3];  // level t at planes (newest-2, newest-1, newest)
+    float iq[T + 1], dq[T + 1];  // inverse and rhs of planes k, k-1, ..., k-T
+#pragma unroll
+    for (int t = 0; t < T; ++t) w[t][0] = w[t][1] = w[t][2] = 0.0f;
+#pragma unroll
+    for (int t = 0; t <= T; ++t) iq[t] = dq[t] = 0.0f;
+
+    float np = 0.0f, nd = 0.0f;
+    uint8_t nf = 0;
+    {
+        const int k = z0 - T;
+        if (in_grid && k >= 0 && k < g.nz) {
+            const int i = base + k * g.plane;
+            np = zero_in ? 0.0f : pin[i];
+            nf = flags[i];
+            nd = d[i];
+        }
+    }
+    for (int k = z0 - T; k < z1 + T; ++k) {
+        const float p0 = np, dd = nd;
+        const uint8_t fl = nf;
+        {
+            const int kn = k + 1;
+            np = 0.0f;
+            nd = 0.0f;
+            nf = 0;
+            if (in_grid && kn >= 0 && kn < g.nz && kn < z1 + T) {
+                const int i = base + kn * g.plane;
+                np = zero_in ? 0.0f : pin[i];
+                nf = flags[i];
+                nd = d[i];
+            }
+        }
+#pragma unroll
+        for (int t = T; t > 0; --t) {
+            iq[t] = iq[t - 1];
+            dq[t] = dq[t - 1];
+        }
+        iq[0] = cell_inv(fl);
+        dq[0] = dd;
+        w[0][0] = w[0][1];
+        w[0][1] = w[0][2];
+        w[0][2] = p0;
+        const int cur = k & 1, prev = cur ^ 1;
+        float newv[T + 1];
+        newv[0] = p0;
+#pragma unroll
+        for (int t = 1; t <= T; ++t) {
+            const float sum = sum6(sh.v[t - 1][prev][ty + 1][tx], sh.v[t - 1][prev][ty + 1][tx + 2],
+                                   sh.v[t - 1][prev][ty][tx + 1], sh.v[t - 1][prev][ty + 2][tx + 1],
+                                   w[t - 1][0], w[t - 1][2]);
+            const float G = (sum - dq[t]) * iq[t];
+            float v;
+            if (DAMPED) {
+                const float pc = w[t - 1][1];
+                v = pc + omega * (G - pc);
+            } else {
+                v = G;
+            }
+            v = (iq[t] > 0.0f) ? v : 0.0f;
+            newv[t] = v;
+            if (t < T) {
+                w[t][0] = w[t][1];
+                w[t][1] = w[t][2];
+                w[t][2] = v;
+            }
+        }
+        const int zo = k - T;
+        if (writes && zo >= z0 && zo < z1) pout[base + zo * g.plane] = newv[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) sh.v[t][cur][ty + 1][tx + 1] = newv[t];
+        __syncthreads();
+    }
+}
+
+// u -= grad P on every face whose two cells are fluid.  Faces that touch no
+// active cell see P_b - P_a = 0 and are skipped (x,y,z >= 1 suffices).
+__device__ __forceinline__ void gradient_at(const Grid& g, float* __restrict__ u,
+                                            float* __restrict__ v, float* __restrict__ w,
+                                            const float* __restrict__ p,
+                                            const uint8_t* __restrict__ flags, int x, int y, int z)
+{
+    if (x < 1 || x > g.nx - 1 || y < 1 || y > g.ny - 1 || z < 1 || z > g.nz - 1) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    if (!(fl & F_SELF)) return;
+    float pc = p[i];
+    if (fl & F_XM) u[i] = u[i] - (pc - p[i - 1]);
+    if (fl & F_YM) v[i] = v[i] - (pc - p[i - g.px]);
+    if (fl & F_ZM) w[i] = w[i] - (pc - p[i - g.plane]);
+}
+
+// avgU/avgV/avgW (smokeSimulation.cu:409-447): 8-point means in the
+// reference's summation order.
+__device__ __forceinline__ float avgU(const float* __restrict__ a, int i, int px, int pl)
+{
+    float t = a[i - pl] + a[i + 1 - pl] + a[i - px - pl] + a[i + 1 - px - pl] + a[i] + a[i + 1] +
+              a[i - px] + a[i + 1 - px];
+    return t / 8.0f;
+}
+__device__ __forceinline__ float avgV(const float* __restrict__ a, int i, int px, int pl)
+{
+    float t = a[i - pl] + a[i - 1 - pl] + a[i + px - pl] + a[i - 1 + px - pl] + a[i] + a[i - 1] +
+              a[i + px] + a[i - 1 + px];
+    return t / 8.0f;
+}
+__device__ __forceinline__ float avgW(const float* __restrict__ a, int i, int px, int pl)
+{
+    float t = a[i] + a[i - 1] + a[i - px] + a[i - 1 - px] + a[i - pl] + a[i - 1 - pl] +
+              a[i - px - pl] + a[i - 1 - px - pl];
+    return t / 8.0f;
+}
+
+// sampleSmoke (smokeSimulation.cu:451-484), split in two so that a kernel can
+// issue the loads of several samples together: Tri holds the clamped
+// trilinear weights and the four row offsets of the corner pairs (x0, x0+1).
+struct Tri {
+    float xw0, xw1, yw0, yw1, zw0, zw1;
+    int r00, r10, r01, r11;  // idx(x0, y0|y1, z0|z1): first index y, second z
+    bool xsame;              // x1 == x0 (position clamped at the last cell)
+};
+
+// Clamp to [1, n-1] with the cell dims and form weights and indices.
+__device__ __forceinline__ Tri tri_setup(const Grid& g, float px_, float py_, float pz_, float dx,
+                                         float dy, float dz)
+{
+    Tri t;
+    float x = fmaxf(fminf(px_, (float)(g.nx - 1)), 1.0f);
+    float y = fmaxf(fminf(py_, (float)(g.ny - 1)), 1.0f);
+    float z = fmaxf(fminf(pz_, (float)(g.nz - 1)), 1.0f);
+    int x0 = (int)fminf(floorf(x - dx), (float)(g.nx - 1));
+    t.xw1 = (x - dx) - (float)x0;
+    t.xw0 = 1.0f - t.xw1;
+    int x1 = (int)fminf((float)(x0 + 1), (float)(g.nx - 1));
+    int y0 = (int)fminf(floorf(y - dy), (float)(g.ny - 1));
+    t.yw1 = (y - dy) - (float)y0;
+    t.yw0 = 1.0f - t.yw1;
+    int y1 = (int)fminf((float)(y0 + 1), (float)(g.ny - 1));
+    int z0 = (int)fminf(floorf(z - dz), (float)(g.nz - 1));
+    t.zw1 = (z - dz) - (float)z0;
+    t.zw0 = 1.0f - t.zw1;
+    int z1 = (int)fminf((float)(z0 + 1), (float)(g.nz - 1));
+    t.r00 = g.idx(x0, y0, z0);
+    t.r10 = g.idx(x0, y1, z0);
+    t.r01 = g.idx(x0, y0, z1);
+    t.r11 = g.idx(x0, y1, z1);
+    t.xsame = x1 == x0;
+    return t;
+}
+
+// Two consecutive floats from a 4-byte-aligned address (one dwordx2 load).
+// a[r+1] exists for every row offset r of a cell x0 <= nx-1: rows are
+// px >= nx+1 long.
+__device__ __forceinline__ float2 ld2(const float* __restrict__ a, int r)
+{
+    float2 v;
+    __builtin_memcpy(&v, a + r, sizeof v);
+    return v;
+}
+
+// The eight corners in the reference's order (:473-481).
+__device__ __forceinline__ float tri_eval(const float* __restrict__ a, const Tri& t)
+{
+    const float2 p00 = ld2(a, t.r00), p10 = ld2(a, t.r10), p01 = ld2(a, t.r01),
+                 p11 = ld2(a, t.r11);
+    const float f000 = p00.x, f100 = t.xsame ? p00.x : p00.y;
+    const float f010 = p10.x, f110 = t.xsame ? p10.x : p10.y;
+    const float f001 = p01.x, f101 = t.xsame ? p01.x : p01.y;
+    const float f011 = p11.x, f111 = t.xsame ? p11.x : p11.y;
+    return t.xw0 * t.yw0 * t.zw0 * f000 + t.xw1 * t.yw0 * t.zw0 * f100 +
+           t.xw0 * t.yw1 * t.zw0 * f010 + t.xw1 * t.yw1 * t.zw0 * f110 +
+           t.xw0 * t.yw0 * t.zw1 * f001 + t.xw1 * t.yw0 * t.zw1 * f101 +
+           t.xw0 * t.yw1 * t.zw1 * f011 + t.xw1 * t.yw1 * t.zw1 * f111;
+}
+
+// velocityAdvectionU/V/W (smokeSimulation.cu:527-615) fused per staggered
+// index over x,y,z in [1,n-1]: each component keeps its own range and
+// fluid-fluid condition; read [now], write [past].  The three averages are
+// shared between the components that use them (avgW feeds both U and V with
+// the same arguments, and so on), and all loads of a phase are issued
+// together: first the own components and the averages, then the corners of
+// all three samples.  Every address read lies inside the allocation (faces up
+// to x = nx, y = ny; clamped sample corners), so the loads are unconditional
+// and only the stores are predicated.
+__device__ __forceinline__ void advect_velocity_at(const Grid& g, const float* __restrict__ u0,
+                                                   const float* __restrict__ v0,
+                                                   const float* __restrict__ w0,
+                                                   float* __restrict__ u1, float* __restrict__ v1,
+                                                   float* __restrict__ w1,
+                                                   const uint8_t* __restrict__ flags, float dt,
+                                                   int x, int y, int z)
+{
+    if (x < 1 || x > g.nx - 1 || y < 1 || y > g.ny - 1 || z < 1 || z > g.nz - 1) return;
+    int i = g.idx(x, y, z);
+    uint8_t fl = flags[i];
+    const bool self = (fl & F_SELF) != 0;
+    const bool doU = self && y < g.ny - 1 && z < g.nz - 1 && (fl & F_XM);
+    const bool doV = self && x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM);
+    const bool doW = self && x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM);
+    if (!(doU || doV || doW)) return;
+    const int px = g.px, pl = g.plane;
+    const float uu = u0[i], vv = v0[i], ww = w0[i];
+    const float aU = avgU(u0, i, px, pl);
+    const float aV = avgV(v0, i, px, pl);
+    const float aW = avgW(w0, i, px, pl);
+    const float xf = (float)x, yf = (float)y, zf = (float)z;
+    const Tri tu = tri_setup(g, xf - dt * uu, (yf + 0.5f) - dt * aV, (zf + 0.5f) - dt * aW, 0.0f,
+                             0.5f, 0.5f);
+    const Tri tv = tri_setup(g, (xf + 0.5f) - dt * aU, yf - dt * vv, (zf + 0.5f) - dt * aW, 0.5f,
+                             0.0f, 0.5f);
+    const Tri tw = tri_setup(g, (xf + 0.5f) - dt * aU, (yf + 0.5f) - dt * aV, zf - dt * ww, 0.5f,
+                             0.5f, 0.0f);
+    const float nu = tri_eval(u0, tu);
+    const float nv = tri_eval(v0, tv);
+    const float nw = tri_eval(w0, tw);
+    if (doU) u1[i] = nu;
+    if (doV) v1[i] = nv;
+    if (doW) w1[i] = nw;
+}
+
+// advectSmoke (smokeSimulation.cu:617-638): cell velocity from the
+// post-advection faces, back-trace in double rounded to float.
+__device__ __forceinline__ void advect_density_at(const Grid& g, const float* __restrict__ r0,
+                                                  float* __restrict__ r1,
+                                                  const float* __restrict__ u,
+                                                  const float* __restrict__ v,
+                                                  const float* __restrict__ w,
+                                                  const uint8_t* __restrict__ flags, float dt,
+                                                  int x, int y, int z)
+{
+    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2) return;
+    int i = g.idx(x, y, z);
+    if (!(flags[i] & F_SELF)) return;
+    float ut = (u[i] + u[i + 1]) / 2.0f;
+    float vt = (v[i] + v[i + g.px]) / 2.0f;
+    float wt = (w[i] + w[i + g.plane]) / 2.0f;
+    float xt = (float)((double)x + 0.5 - (double)(ut * dt));
+    float yt = (float)((double)y + 0.5 - (double)(vt * dt));
+    float zt = (float)((double)z + 0.5 - (double)(wt * dt));
+    r1[i] = tri_eval(r0, tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f));
+}
+
+}  // namespace dev
+}  // namespace smk

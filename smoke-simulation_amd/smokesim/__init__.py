@@ -116,6 +116,15 @@ def _declare(lib):
         "smoke_timing_reset": (st, [vp]),
         "smoke_timing_step_ms": (st, [vp, dp, i, ip]),
         "smoke_timing_sweeps": (st, [vp, dp, llp]),
+        "smoke_batch_create": (st, [ctypes.POINTER(vp), i, fp, ctypes.POINTER(vp)]),
+        "smoke_batch_destroy": (None, [vp]),
+        "smoke_batch_step": (st, [vp, i]),
+        "smoke_batch_synchronize": (st, [vp]),
+        "smoke_batch_set_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
+        "smoke_batch_get_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
+        "smoke_batch_autotune": (st, [vp, ctypes.POINTER(JacobiPlan), dp]),
+        "smoke_batch_step_ms": (st, [vp, dp, i, ip]),
+        "smoke_batch_cells": (ctypes.c_longlong, [vp]),
         "smoke_legacy_device_info": (st, []),
         "smoke_legacy_initialize": (st, [fp, u, u, u]),
         "smoke_legacy_delete": (st, []),
@@ -393,3 +402,67 @@ class Simulation:
             for k, name in enumerate("uvw"):
                 out[f"{name}{h}"] = self.get_velocity(k, h)
         return out
+
+
+class Batch:
+    """Simulations advanced together, one launch per stage (smoke_batch_*)."""
+
+    def __init__(self, sims, dts):
+        self.lib = load()
+        self.sims = list(sims)  # keeps the members alive
+        n = len(self.sims)
+        handles = (ctypes.c_void_p * n)(*[s.h.value for s in self.sims])
+        d = np.ascontiguousarray(dts, dtype=np.float32)
+        if d.shape != (n,):
+            raise ValueError("one dt per simulation")
+        h = ctypes.c_void_p()
+        _check(self.lib.smoke_batch_create(handles, n, _fptr(d), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.smoke_batch_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def cells(self):
+        return int(self.lib.smoke_batch_cells(self.h))
+
+    def step(self, use_graph=False):
+        _check(self.lib.smoke_batch_step(self.h, 1 if use_graph else 0))
+
+    def synchronize(self):
+        _check(self.lib.smoke_batch_synchronize(self.h))
+
+    def get_jacobi_plan(self):
+        p = JacobiPlan()
+        _check(self.lib.smoke_batch_get_jacobi_plan(self.h, ctypes.byref(p)))
+        return p.as_dict()
+
+    def set_jacobi_plan(self, **kw):
+        cur = self.get_jacobi_plan()
+        cur.update(kw)
+        p = JacobiPlan(**cur)
+        _check(self.lib.smoke_batch_set_jacobi_plan(self.h, ctypes.byref(p)))
+
+    def autotune(self):
+        p = JacobiPlan()
+        ms = ctypes.c_double()
+        _check(self.lib.smoke_batch_autotune(self.h, ctypes.byref(p), ctypes.byref(ms)))
+        return p.as_dict(), ms.value
+
+    def step_ms(self, cap=1 << 16):
+        buf = (ctypes.c_double * cap)()
+        n = ctypes.c_int()
+        _check(self.lib.smoke_batch_step_ms(self.h, buf, cap, ctypes.byref(n)))
+        return np.array(buf[: n.value], dtype=np.float64)

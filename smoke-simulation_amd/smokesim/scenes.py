@@ -168,6 +168,29 @@ def grid_512(projection=2, iterations=100):
     return grid(512, 8, "grid_512", projection, iterations)
 
 
+def batch_member(sim_id, workload="batch_64", lo=32, hi=160, iterations=60):
+    """One simulation of the batched workload: each axis uniform in [lo, hi],
+    seeded boxes covering a uniform 0-10 % of cells, 1-3 sources (r in
+    [3, 8]), dt uniform in [0.02, 0.08], Jacobi x iterations."""
+    rng = Stream(stream_seed(workload, sim_id, PURPOSE_DIMS))
+    dims = (rng.integer(lo, hi), rng.integer(lo, hi), rng.integer(lo, hi))
+    frac = rng.uniform(0.0, 0.10)
+    n_src = rng.integer(1, 3)
+    dt = float(np.float32(rng.uniform(0.02, 0.08)))
+    solid, boxes = seeded_boxes(dims, frac, stream_seed(workload, sim_id, PURPOSE_BOXES))
+    return dict(
+        name=f"{workload}[{sim_id}]", dims=dims, solid=solid, boxes=boxes, obstacles=[],
+        sources=seeded_sources(dims, n_src, stream_seed(workload, sim_id, PURPOSE_SOURCES),
+                               radius=(3.0, 8.0)),
+        velocity=velocity_noise(dims, 0.01, stream_seed(workload, sim_id, PURPOSE_VELOCITY)),
+        params=dict(PHYSICS, projection=2, iterations=iterations), dt=dt, steps=50)
+
+
+def batch_64(count=64, iterations=60):
+    """The batched workload: `count` independent simulations."""
+    return [batch_member(i, iterations=iterations) for i in range(count)]
+
+
 def test_scene(dims, seed=0, box_fraction=0.05, n_sources=2, n_obstacles=1, projection=0,
                iterations=20, noise=0.01):
     """Small randomised scene for parity tests."""
