@@ -282,6 +282,59 @@ smoke_status smoke_batch_step_ms(smoke_batch* batch, double* ms, int cap, int* c
 /* Cells of all simulations of the batch (for cell-updates/s). */
 long long smoke_batch_cells(const smoke_batch* batch);
 
+/* --------------------------------------------------------------- serving */
+/* A request loop over a stream of independent simulation requests
+ * (continuous batching): each tick admits queued requests in arrival order
+ * while their device footprint fits the HBM budget, steps every resident
+ * request once in one batch (one launch per stage; replayed from a HIP graph
+ * while the membership is unchanged), and retires requests that have run all
+ * their steps (their final density is copied out and their memory freed).
+ * A request's result is the same bits alone and in any mix.  No reference
+ * counterpart.  All requests use the Jacobi projection with the server's
+ * iteration count. */
+typedef struct smoke_server smoke_server;
+typedef struct smoke_request {
+    unsigned nx, ny, nz;
+    int steps;              /* >= 1 */
+    float dt;
+    float gravity, buoyancy, density_gate, max_velocity;
+    const uint8_t* solid;   /* nx*ny*nz bytes, nonzero = solid; NULL: floor only */
+    int n_sources;          /* 0..8 */
+    const float* sources;   /* x, y, z, r per source */
+    int n_obstacles;        /* 0..SMOKE_MAX_OBJECTS */
+    const float* obstacles; /* x, y, z, r per obstacle */
+    const float* velocity[3]; /* initial u, v, w ((nx+1)(ny+1)(nz+1)); NULL: zero */
+} smoke_request;
+typedef enum smoke_request_state {
+    SMOKE_REQUEST_QUEUED = 0,
+    SMOKE_REQUEST_RUNNING = 1,
+    SMOKE_REQUEST_DONE = 2
+} smoke_request_state;
+typedef struct smoke_server_stats {
+    long long submitted, admitted, completed, ticks;
+    long long cell_updates;  /* cells x steps of all completed and running work */
+    double gpu_ms;           /* GPU time of all batched steps */
+    size_t hbm_used, hbm_budget, hbm_peak;
+    int active, queued;
+} smoke_server_stats;
+smoke_status smoke_server_create(size_t hbm_budget_bytes, int iterations, smoke_server** out);
+void smoke_server_destroy(smoke_server* server);
+/* Copies the request's inputs and queues it; *id numbers requests from 0. */
+smoke_status smoke_server_submit(smoke_server* server, const smoke_request* req, long long* id);
+/* One scheduler iteration (admit, step, retire).  Returns how many requests
+ * are running and queued after it. */
+smoke_status smoke_server_tick(smoke_server* server, int* active, int* queued);
+/* State of a request; when DONE and density is not NULL, copies its final
+ * density (nx*ny*nz floats).  queue_ms / run_ms (may be NULL): host time from
+ * submission to admission and from admission to completion. */
+smoke_status smoke_server_result(smoke_server* server, long long id, int* state, float* density,
+                                 double* queue_ms, double* run_ms);
+smoke_status smoke_server_stats_get(smoke_server* server, smoke_server_stats* out);
+/* GPU time (ms) of each of the last `cap` ticks' batched step and the number
+ * of requests it advanced, oldest first. */
+smoke_status smoke_server_tick_ms(smoke_server* server, double* ms, int* requests, int cap,
+                                  int* count);
+
 /* ------------------------------------------------------ legacy interface */
 /* Process-global single simulation behind the reference's nine functions
  * (smokeSimulation.cuh:4-17).  Used by smokeSimulation.cuh; errors are

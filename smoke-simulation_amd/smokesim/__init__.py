@@ -125,6 +125,13 @@ def _declare(lib):
         "smoke_batch_autotune": (st, [vp, ctypes.POINTER(JacobiPlan), dp]),
         "smoke_batch_step_ms": (st, [vp, dp, i, ip]),
         "smoke_batch_cells": (ctypes.c_longlong, [vp]),
+        "smoke_server_create": (st, [ctypes.c_size_t, i, ctypes.POINTER(vp)]),
+        "smoke_server_destroy": (None, [vp]),
+        "smoke_server_submit": (st, [vp, ctypes.POINTER(Request), llp]),
+        "smoke_server_tick": (st, [vp, ip, ip]),
+        "smoke_server_result": (st, [vp, ctypes.c_longlong, ip, fp, dp, dp]),
+        "smoke_server_stats_get": (st, [vp, ctypes.POINTER(ServerStats)]),
+        "smoke_server_tick_ms": (st, [vp, dp, ip, i, ip]),
         "smoke_legacy_device_info": (st, []),
         "smoke_legacy_initialize": (st, [fp, u, u, u]),
         "smoke_legacy_delete": (st, []),
@@ -466,3 +473,112 @@ class Batch:
         n = ctypes.c_int()
         _check(self.lib.smoke_batch_step_ms(self.h, buf, cap, ctypes.byref(n)))
         return np.array(buf[: n.value], dtype=np.float64)
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint), ("ny", ctypes.c_uint), ("nz", ctypes.c_uint),
+        ("steps", ctypes.c_int), ("dt", ctypes.c_float),
+        ("gravity", ctypes.c_float), ("buoyancy", ctypes.c_float),
+        ("density_gate", ctypes.c_float), ("max_velocity", ctypes.c_float),
+        ("solid", ctypes.POINTER(ctypes.c_uint8)),
+        ("n_sources", ctypes.c_int), ("sources", ctypes.POINTER(ctypes.c_float)),
+        ("n_obstacles", ctypes.c_int), ("obstacles", ctypes.POINTER(ctypes.c_float)),
+        ("velocity", ctypes.POINTER(ctypes.c_float) * 3),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_longlong), ("admitted", ctypes.c_longlong),
+        ("completed", ctypes.c_longlong), ("ticks", ctypes.c_longlong),
+        ("cell_updates", ctypes.c_longlong), ("gpu_ms", ctypes.c_double),
+        ("hbm_used", ctypes.c_size_t), ("hbm_budget", ctypes.c_size_t),
+        ("hbm_peak", ctypes.c_size_t), ("active", ctypes.c_int), ("queued", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQUEST_QUEUED, REQUEST_RUNNING, REQUEST_DONE = 0, 1, 2
+
+
+class Server:
+    """Request loop with continuous batching (smoke_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, iterations):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        _check(self.lib.smoke_server_create(int(hbm_budget_bytes), int(iterations), ctypes.byref(h)))
+        self.h = h
+        self.dims = {}
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.smoke_server_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, scene, steps):
+        """Queue a scene (smokesim.scenes format: dims, solid, sources,
+        obstacles, velocity, params, dt) for `steps` steps; returns its id."""
+        nx, ny, nz = scene["dims"]
+        p = scene["params"]
+        keep = []
+        q = Request()
+        q.nx, q.ny, q.nz, q.steps, q.dt = nx, ny, nz, int(steps), float(scene["dt"])
+        q.gravity, q.buoyancy = p["gravity"], p["buoyancy"]
+        q.density_gate, q.max_velocity = p["density_gate"], p["max_velocity"]
+        if scene.get("solid") is not None:
+            a = np.ascontiguousarray(scene["solid"], dtype=np.uint8)
+            keep.append(a)
+            q.solid = _u8ptr(a)
+        for field, cnt in (("sources", "n_sources"), ("obstacles", "n_obstacles")):
+            items = scene.get(field) or []
+            if items:
+                a = np.ascontiguousarray(np.array(items, dtype=np.float32).reshape(-1, 4))
+                keep.append(a)
+                setattr(q, field, _fptr(a))
+            setattr(q, cnt, len(items))
+        for k, v in enumerate(scene.get("velocity") or []):
+            a = np.ascontiguousarray(v, dtype=np.float32)
+            keep.append(a)
+            q.velocity[k] = _fptr(a)
+        rid = ctypes.c_longlong()
+        _check(self.lib.smoke_server_submit(self.h, ctypes.byref(q), ctypes.byref(rid)))
+        self.dims[rid.value] = (nz, ny, nx)
+        return rid.value
+
+    def tick(self):
+        a, q = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.smoke_server_tick(self.h, ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def result(self, rid):
+        """(state, density or None, queue_ms, run_ms)."""
+        st = ctypes.c_int()
+        qms, rms = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.smoke_server_result(self.h, rid, ctypes.byref(st), None,
+                                            ctypes.byref(qms), ctypes.byref(rms)))
+        if st.value != REQUEST_DONE:
+            return st.value, None, None, None
+        out = np.empty(self.dims[rid], dtype=np.float32)
+        _check(self.lib.smoke_server_result(self.h, rid, ctypes.byref(st), _fptr(out), None, None))
+        return st.value, out, qms.value, rms.value
+
+    def stats(self):
+        s = ServerStats()
+        _check(self.lib.smoke_server_stats_get(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def tick_ms(self, cap=1 << 20):
+        ms = (ctypes.c_double * cap)()
+        req = (ctypes.c_int * cap)()
+        n = ctypes.c_int()
+        _check(self.lib.smoke_server_tick_ms(self.h, ms, req, cap, ctypes.byref(n)))
+        return (np.array(ms[: n.value], dtype=np.float64),
+                np.array(req[: n.value], dtype=np.int64))

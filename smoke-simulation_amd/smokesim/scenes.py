@@ -186,6 +186,23 @@ def batch_member(sim_id, workload="batch_64", lo=32, hi=160, iterations=60):
         params=dict(PHYSICS, projection=2, iterations=iterations), dt=dt, steps=50)
 
 
+def serve_request(req_id, workload="serve", lo=48, hi=256):
+    """One request of the serving workload: each axis log-uniform in
+    [lo, hi], 20-200 steps, 1-3 sources in the lower third, solid floor,
+    starting at rest (no noise, so requests can be generated as they arrive),
+    dt = 0.05.  Returns (scene, steps)."""
+    rng = Stream(stream_seed(workload, req_id, PURPOSE_DIMS))
+    dims = tuple(int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(3))
+    steps = rng.integer(20, 200)
+    n_src = rng.integer(1, 3)
+    sc = dict(
+        name=f"{workload}[{req_id}]", dims=dims, solid=None, obstacles=[],
+        sources=seeded_sources(dims, n_src, stream_seed(workload, req_id, PURPOSE_SOURCES),
+                               radius=(3.0, 8.0)),
+        velocity=[], params=dict(PHYSICS, projection=2, iterations=60), dt=0.05)
+    return sc, steps
+
+
 def batch_64(count=64, iterations=60):
     """The batched workload: `count` independent simulations."""
     return [batch_member(i, iterations=iterations) for i in range(count)]

@@ -1,0 +1,165 @@
+"""Test harness: parity cases run against the CPU-emulated build of libsmokesim
+(tests/emu/Makefile) under AddressSanitizer, compared bit for bit with the
+FP32 oracle.  Run as a child process with the ASan runtime preloaded:
+
+    python emu_cases.py <path to libsmokesim_emu.so> [case ...]
+
+Every kernel access is checked against its allocation, so an out-of-bounds
+index shows up here, on the CPU, with a stack trace.  Prints "ok <case>" per
+case and exits non-zero on the first failure."""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "smoke-simulation_amd"), os.path.join(ROOT, "tests"),
+                os.path.join(ROOT, "oracle"), ROOT]
+
+import numpy as np  # noqa: E402
+
+import smokesim  # noqa: E402
+from helpers import assert_bitwise_equal, gpu_for, oracle_for  # noqa: E402
+from smokesim import scenes  # noqa: E402
+
+
+def same_bits(a, b, what):
+    assert np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                          np.ascontiguousarray(b, dtype=np.float32).view(np.uint32)), what
+
+
+def case_single():
+    for proj, its, dims, seed in [(0, 6, (15, 13, 11), 1), (1, 5, (12, 16, 9), 2),
+                                  (2, 9, (17, 11, 13), 3)]:
+        sc = scenes.test_scene(dims, seed=seed, projection=proj, iterations=its)
+        g, o = gpu_for(sc), oracle_for(sc, 32)
+        for _ in range(2):
+            g.step(sc["dt"], readback=False)
+        o.step(sc["dt"], 2)
+        assert_bitwise_equal(g.state(), o.state(), f"single p{proj}")
+        g.close()
+
+
+def case_plans():
+    sc = scenes.test_scene((19, 14, 12), seed=4, projection=2, iterations=7)
+    o = oracle_for(sc, 32)
+    o.step(sc["dt"], 1)
+    for t, tile, zc in [(3, 1, 7), (6, 0, 4), (2, 1, 64)]:
+        g = gpu_for(sc)
+        g.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
+        g.step(sc["dt"], readback=False)
+        assert_bitwise_equal(g.state(), o.state(), f"plan {(t, tile, zc)}")
+        g.close()
+
+
+def case_batch():
+    dims = [(13, 11, 9), (9, 15, 12), (16, 8, 10)]
+    scs = []
+    for k, d in enumerate(dims):
+        sc = scenes.test_scene(d, seed=20 + k, projection=2, iterations=6, n_sources=1 + k % 2,
+                               n_obstacles=k % 2)
+        sc["dt"] = 0.03 + 0.01 * k
+        scs.append(sc)
+    sims = [gpu_for(sc) for sc in scs]
+    with smokesim.Batch(sims, [sc["dt"] for sc in scs]) as b:
+        b.step(use_graph=False)
+        for _ in range(2):
+            b.step(use_graph=True)
+        b.synchronize()
+        states = [s.state() for s in sims]
+    for sc, st in zip(scs, states):
+        o = oracle_for(sc, 32)
+        o.step(sc["dt"], 3)
+        assert_bitwise_equal(st, o.state(), f"batch {sc['name']}")
+    for s in sims:
+        s.close()
+
+
+def _req(dims, seed, dt, its=6):
+    sc = scenes.test_scene(dims, seed=seed, projection=2, iterations=its, n_sources=2,
+                           n_obstacles=1)
+    sc["dt"] = dt
+    return sc
+
+
+def _drain(srv):
+    for _ in range(200):
+        a, q = srv.tick()
+        if a == 0 and q == 0:
+            return
+    raise AssertionError("server did not drain")
+
+
+def case_server():
+    a = _req((14, 12, 10), 60, 0.05)
+    others = [(_req((16, 10, 13), 61, 0.03), 5), (_req((9, 15, 8), 62, 0.07), 2),
+              (_req((12, 12, 12), 63, 0.04), 3)]
+    alone = smokesim.Server(1 << 30, 6)
+    ia = alone.submit(a, 4)
+    _drain(alone)
+    _, d_alone, _, _ = alone.result(ia)
+    alone.close()
+    mix = smokesim.Server(1 << 30, 6)
+    ids = [mix.submit(others[0][0], others[0][1]), mix.submit(others[1][0], others[1][1])]
+    mix.tick()
+    ia2 = mix.submit(a, 4)           # arrives while others run
+    mix.tick()
+    ids.append(mix.submit(others[2][0], others[2][1]))
+    _drain(mix)
+    _, d_mix, _, _ = mix.result(ia2)
+    same_bits(d_mix, d_alone, "server: alone vs mix")
+    o = oracle_for(a, 32)
+    o.step(a["dt"], 4)
+    same_bits(d_mix, o.output_density(), "server: mix vs oracle")
+    for rid, (sc, steps) in zip(ids, others):
+        _, d, _, _ = mix.result(rid)
+        o = oracle_for(sc, 32)
+        o.step(sc["dt"], steps)
+        same_bits(d, o.output_density(), f"server: {sc['name']}")
+    s = mix.stats()
+    assert s["completed"] == 4 and s["hbm_used"] == 0, s
+    ms, nreq = mix.tick_ms()
+    assert len(ms) == s["ticks"] and nreq.max() == 3, (len(ms), s["ticks"], nreq)
+    mix.close()
+    # admission under a budget that holds two requests
+    fp = smokesim.footprint_bytes(12, 12, 12)
+    srv = smokesim.Server(int(2.5 * fp), 6)
+    scs = [_req((12, 12, 12), 70 + k, 0.05) for k in range(4)]
+    rids = [srv.submit(sc, 2) for sc in scs]
+    assert srv.tick() == (2, 2)
+    _drain(srv)
+    st = srv.stats()
+    assert st["completed"] == 4 and st["hbm_peak"] <= st["hbm_budget"], st
+    for rid, sc in zip(rids, scs):
+        _, d, _, _ = srv.result(rid)
+        o = oracle_for(sc, 32)
+        o.step(sc["dt"], 2)
+        same_bits(d, o.output_density(), f"budgeted {sc['name']}")
+    srv.close()
+
+
+def case_readback():
+    sc = scenes.test_scene((13, 12, 11), seed=5, projection=2, iterations=5)
+    g = gpu_for(sc)
+    for k in range(3):
+        g.step_async(sc["dt"])
+        g.readback_begin(smokesim.READBACK_F32)
+        host, step = g.readback_wait()
+        assert step == k + 1
+        same_bits(host, g.read_density(), "readback f32")
+    g.close()
+
+
+CASES = {"single": case_single, "plans": case_plans, "batch": case_batch,
+         "server": case_server, "readback": case_readback}
+
+
+def main():
+    smokesim.LIB_PATH = sys.argv[1]
+    names = sys.argv[2:] or list(CASES)
+    for n in names:
+        CASES[n]()
+        print("ok", n, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,301 @@
+// Test harness: the subset of the HIP runtime API that libsmokesim's host code
+// calls, implemented on the CPU, plus the kernel emulator behind
+// hipLaunchKernelGGL (emu_shim.h).
+//
+//  * "Device" allocations are plain heap blocks (AddressSanitizer bounds).
+//  * Work runs synchronously in submission order; a stream under capture
+//    records launches instead, and a graph launch replays them.
+//  * Kernels without barriers run their threads one after another; the
+//    fused-Jacobi kernels (the only ones with __syncthreads) run every thread
+//    of a block on its own host thread with a real barrier, one block at a
+//    time, which is what makes `static` stand in for LDS.
+//  * Handles carry magic numbers and are checked on every use, so a call on
+//    a destroyed stream, event or graph stops the run.
+#include <hip/hip_runtime_api.h>
+
+#include <barrier>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <set>
+#include <thread>
+#include <vector>
+
+namespace {
+constexpr unsigned kStream = 0x5354524dU, kEvent = 0x4556454eU, kGraph = 0x47524150U,
+                   kExec = 0x45584543U;
+using Op = std::function<void()>;
+}  // namespace
+
+struct ihipStream_t {
+    unsigned magic;
+    bool capturing;
+    std::vector<Op> rec;
+};
+struct ihipEvent_t {
+    unsigned magic;
+};
+struct ihipGraph {
+    unsigned magic;
+    std::vector<Op> ops;
+};
+struct hipGraphExec {
+    unsigned magic;
+    std::vector<Op> ops;
+};
+
+struct EmuIdx {
+    unsigned x, y, z;
+};
+thread_local EmuIdx threadIdx, blockIdx, blockDim, gridDim;
+
+namespace {
+
+std::set<void*> g_device;
+std::barrier<>* g_barrier = nullptr;
+bool g_threaded = false;
+
+[[noreturn]] void die(const char* what)
+{
+    fprintf(stderr, "emu: %s\n", what);
+    abort();
+}
+void check(hipStream_t s)
+{
+    if (s && s->magic != kStream) die("use of an invalid stream");
+}
+void check(hipEvent_t e)
+{
+    if (!e || e->magic != kEvent) die("use of an invalid event");
+}
+
+void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 block)
+{
+    const bool threaded = strstr(name, "jacobi_fused") || strstr(name, "kb_jacobi<");
+    const unsigned nt = block.x * block.y * block.z;
+    const EmuIdx gd{grid.x, grid.y, grid.z}, bd{block.x, block.y, block.z};
+    auto set_thread = [&](unsigned t) {
+        threadIdx = {t % block.x, (t / block.x) % block.y, t / (block.x * block.y)};
+    };
+    if (!threaded) {
+        blockDim = bd;
+        gridDim = gd;
+        for (unsigned bz = 0; bz < grid.z; ++bz)
+            for (unsigned by = 0; by < grid.y; ++by)
+                for (unsigned bx = 0; bx < grid.x; ++bx) {
+                    blockIdx = {bx, by, bz};
+                    for (unsigned t = 0; t < nt; ++t) {
+                        set_thread(t);
+                        fn();
+                    }
+                }
+        return;
+    }
+    std::barrier<> bar((std::ptrdiff_t)nt);
+    g_barrier = &bar;
+    g_threaded = true;
+    std::vector<std::thread> threads;
+    threads.reserve(nt);
+    for (unsigned t = 0; t < nt; ++t)
+        threads.emplace_back([&, t]() {
+            blockDim = bd;
+            gridDim = gd;
+            set_thread(t);
+            for (unsigned bz = 0; bz < grid.z; ++bz)
+                for (unsigned by = 0; by < grid.y; ++by)
+                    for (unsigned bx = 0; bx < grid.x; ++bx) {
+                        blockIdx = {bx, by, bz};
+                        fn();
+                        bar.arrive_and_wait();  // the block ends before the next starts
+                    }
+        });
+    for (std::thread& t : threads) t.join();
+    g_threaded = false;
+    g_barrier = nullptr;
+}
+
+}  // namespace
+
+void emu_syncthreads()
+{
+    if (!g_threaded) die("__syncthreads in a kernel run without threads");
+    g_barrier->arrive_and_wait();
+}
+
+hipError_t emu_last_error() { return hipSuccess; }
+
+void emu_launch(const char* name, std::function<void()> fn, dim3 grid, dim3 block, hipStream_t st)
+{
+    check(st);
+    if (st && st->capturing) {
+        st->rec.push_back([=]() { run(name, fn, grid, block); });
+        return;
+    }
+    run(name, fn, grid, block);
+}
+
+extern "C" {
+
+hipError_t hipMalloc(void** p, size_t n)
+{
+    *p = malloc(n ? n : 1);
+    g_device.insert(*p);
+    return hipSuccess;
+}
+hipError_t hipFree(void* p)
+{
+    if (!p) return hipSuccess;
+    if (!g_device.erase(p)) die("hipFree of a pointer hipMalloc did not return");
+    free(p);
+    return hipSuccess;
+}
+hipError_t hipHostMalloc(void** p, size_t n, unsigned)
+{
+    *p = malloc(n ? n : 1);
+    return hipSuccess;
+}
+hipError_t hipHostFree(void* p)
+{
+    free(p);
+    return hipSuccess;
+}
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind)
+{
+    memcpy(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t st)
+{
+    check(st);
+    if (st && st->capturing) die("copy recorded into a graph");
+    memcpy(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipGetDevice(int* d)
+{
+    *d = 0;
+    return hipSuccess;
+}
+hipError_t hipGetDeviceCount(int* c)
+{
+    *c = 1;
+    return hipSuccess;
+}
+hipError_t hipGetDevicePropertiesR0600(hipDeviceProp_tR0600* p, int)
+{
+    memset(p, 0, sizeof *p);
+    strcpy(p->name, "cpu emulator");
+    strcpy(p->gcnArchName, "emu");
+    p->multiProcessorCount = 256;
+    p->totalGlobalMem = 288ull << 30;
+    return hipSuccess;
+}
+const char* hipGetErrorString(hipError_t) { return "emulator error"; }
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned)
+{
+    *s = new ihipStream_t{kStream, false, {}};
+    return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t s)
+{
+    check(s);
+    s->magic = 0;
+    delete s;
+    return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t s)
+{
+    check(s);
+    if (s && s->capturing) die("synchronize on a capturing stream");
+    return hipSuccess;
+}
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned)
+{
+    check(s);
+    check(e);
+    return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode)
+{
+    check(s);
+    s->capturing = true;
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g)
+{
+    check(s);
+    s->capturing = false;
+    *g = new ihipGraph{kGraph, std::move(s->rec)};
+    s->rec.clear();
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t)
+{
+    if (!g || g->magic != kGraph) die("instantiate of an invalid graph");
+    *e = new hipGraphExec{kExec, g->ops};
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s)
+{
+    check(s);
+    if (!e || e->magic != kExec) die("launch of an invalid graph");
+    for (Op& op : e->ops) op();
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e)
+{
+    if (!e || e->magic != kExec) die("destroy of an invalid graph exec");
+    e->magic = 0;
+    delete e;
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g)
+{
+    if (!g || g->magic != kGraph) die("destroy of an invalid graph");
+    g->magic = 0;
+    delete g;
+    return hipSuccess;
+}
+hipError_t hipEventCreate(hipEvent_t* e)
+{
+    *e = new ihipEvent_t{kEvent};
+    return hipSuccess;
+}
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned)
+{
+    *e = new ihipEvent_t{kEvent};
+    return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e)
+{
+    check(e);
+    e->magic = 0;
+    delete e;
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s)
+{
+    check(e);
+    check(s);
+    return hipSuccess;
+}
+hipError_t hipEventSynchronize(hipEvent_t e)
+{
+    check(e);
+    return hipSuccess;
+}
+hipError_t hipEventQuery(hipEvent_t e)
+{
+    check(e);
+    return hipSuccess;
+}
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b)
+{
+    check(a);
+    check(b);
+    *ms = 1.0f;
+    return hipSuccess;
+}
+
+}  // extern "C"
