@@ -42,12 +42,26 @@ typedef enum smoke_status {
 /* Pressure projection.  REFERENCE is the reference's in-place red-black SOR on
  * face velocities (smokeSimulation.cu:356-394, :795-801).  RBSOR and JACOBI
  * solve the same linear system in pressure form (P, right-hand side div u*,
- * then u -= grad P on fluid-fluid faces). */
+ * then u -= grad P on fluid-fluid faces).  REFINED is the mixed-precision
+ * solve of that system: an FP32 residual-correction loop whose corrections
+ * take `inner_sweeps` FP16 Jacobi sweeps each, FP64 norms on the device; it
+ * runs ceil(iterations / inner_sweeps) refinements, fewer once the relative
+ * residual reaches `tolerance` (> 0).  Cold start only. */
 typedef enum smoke_projection {
     SMOKE_PROJ_REFERENCE = 0,
     SMOKE_PROJ_RBSOR = 1,
-    SMOKE_PROJ_JACOBI = 2
+    SMOKE_PROJ_JACOBI = 2,
+    SMOKE_PROJ_REFINED = 3
 } smoke_projection;
+
+/* Storage of density and u, v, w (chosen at creation).  BF16 keeps FP32
+ * arithmetic (widen on load, round to nearest even on store); pressure and
+ * divergence stay FP32.  BF16 simulations use the pressure forms (RBSOR,
+ * JACOBI, REFINED), not REFERENCE, and do not join batches. */
+typedef enum smoke_storage {
+    SMOKE_STORAGE_FP32 = 0,
+    SMOKE_STORAGE_BF16 = 1
+} smoke_storage;
 
 /* How several obstacles combine into the interior mask.  LAST_WINS is the
  * reference's fillObstacle (smokeSimulation.cu:295-311); UNION marks a cell
@@ -70,6 +84,10 @@ typedef struct smoke_params {
                            (two half-sweeps each) or Jacobi sweeps            */
     int obstacle_mode;  /* smoke_obstacle_mode, default LAST_WINS             */
     int warm_start;     /* pressure forms: start from the last step's P       */
+    int inner_sweeps;   /* REFINED: FP16 sweeps per refinement (2..1000, even),
+                           default 20                                         */
+    int reserved;       /* 0                                                  */
+    double tolerance;   /* REFINED: stop at this relative residual; 0: run all */
 } smoke_params;
 
 /* Step stages, in execution order (for stage-level testing and timing). */
@@ -108,6 +126,12 @@ smoke_status smoke_params_default(smoke_params* out);
 smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* density,
                           smoke_sim** out);
 
+/* smoke_create with a field storage (smoke_storage).  No reference
+ * counterpart: the mixed-precision form's BF16 fields. */
+smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float* density,
+                             int storage, smoke_sim** out);
+int smoke_get_storage(const smoke_sim* sim);
+
 /* Replaces deleteVolume (smokeSimulation.cu:240-248).  NULL is a no-op. */
 void smoke_destroy(smoke_sim* sim);
 
@@ -115,6 +139,12 @@ smoke_status smoke_get_dims(const smoke_sim* sim, unsigned* nx, unsigned* ny, un
 
 /* Device bytes held by one simulation of the given size (budgeting). */
 size_t smoke_footprint_bytes(unsigned nx, unsigned ny, unsigned nz);
+/* ... with a storage and, if `refined`, the REFINED solver's buffers. */
+size_t smoke_footprint_bytes_ex(unsigned nx, unsigned ny, unsigned nz, int storage, int refined);
+
+/* REFINED: relative residual ||r|| / ||div|| (FP64) after the last step's
+ * solve and the refinements it ran.  Waits for the simulation's work. */
+smoke_status smoke_get_solve_info(smoke_sim* sim, double* rel_residual, int* refinements);
 
 /* Replaces the globals behind getGravity/getBuoyancy (smokeSimulation.cu:28-36)
  * and the compile-time constants (:38-39, :797). */

@@ -1,0 +1,431 @@
+// Mixed-precision pressure solve for gfx950: an FP32 residual-correction loop
+// around FP16 Jacobi sweeps, with the global norms reduced in FP64.
+//
+// The pressure equation is the Jacobi form's acc*P - sum_n P_n = -d on the
+// active cells (d = divergence).  With r = d + acc*x - sum_n x_n (the negated
+// residual of the current FP32 iterate x), one refinement
+//   1. scales r by s = 2^k so that max|r*s| lies in [1/2, 1),
+//   2. runs n Jacobi sweeps on acc*e - sum_n e_n = -r*s from e = 0 in packed
+//      FP16 (two cells per 32-bit word, v_pk_* arithmetic):
+//        e <- inv * sum_n e_n - inv*r*s,   inv = 1/acc (0 off the active set),
+//   3. adds x += e / s in FP32 and forms the next r (one fused pass).
+// In exact arithmetic refinement with Jacobi inner sweeps from e = 0 is the
+// plain Jacobi iteration (x + e_k obeys the same recurrence as x_k), so R
+// refinements of n sweeps equal R*n Jacobi sweeps; only the rounding differs.
+// |e| <= n * max|inv*r*s| <= n, far inside FP16's range.
+//
+// Norms: every residual pass writes one FP64 partial sum of r^2 and one
+// max|r| per block; a one-block kernel reduces them in a fixed order (so the
+// decision is deterministic), sets the scale of the next correction and, with
+// a tolerance, the `done` flag once ||r|| / ||d|| <= tol.  Later kernels of the
+// step read `done` and idle, so the loop needs no host round trip and can be
+// captured.
+#include "step_device.h"
+
+namespace smk {
+
+namespace {
+
+using dev::BX;
+using dev::BY;
+
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2 ld_h2(const uint32_t* __restrict__ p, int w)
+{
+    return __builtin_bit_cast(h2, p[w]);
+}
+__device__ __forceinline__ void st_h2(uint32_t* __restrict__ p, int w, h2 v)
+{
+    p[w] = __builtin_bit_cast(uint32_t, v);
+}
+__device__ __forceinline__ float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+
+// Diagonal inverse per cell as FP16 (0 off the active set).
+__global__ void k_ref_inv(Grid g, const uint8_t* __restrict__ flags, uint16_t* __restrict__ inv)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.px || y > g.ny || z > g.nz) return;
+    int i = g.idx(x, y, z);
+    const float v = (x < g.nx && y < g.ny && z < g.nz) ? dev::cell_inv(flags[i]) : 0.0f;
+    inv[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+// Block reduction of an FP64 sum and an FP32 max (blocks of BX*BY threads), in
+// a fixed order; thread 0 stores the block's pair.
+__device__ __forceinline__ void block_partials(double s2, float m, double* __restrict__ part,
+                                               unsigned* __restrict__ pmax)
+{
+    __shared__ double ss[BX * BY / 64];
+    __shared__ float sm[BX * BY / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        s2 += __shfl_xor(s2, off, 64);
+        m = fmaxf(m, __shfl_xor(m, off, 64));
+    }
+    const int t = (int)(threadIdx.x + blockDim.x * threadIdx.y);
+    if ((t & 63) == 0) {
+        ss[t >> 6] = s2;
+        sm[t >> 6] = m;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double a = 0.0;
+        float b = 0.0f;
+        for (int k = 0; k < BX * BY / 64; ++k) {
+            a += ss[k];
+            b = fmaxf(b, sm[k]);
+        }
+        const int blk = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+        part[blk] = a;
+        pmax[blk] = __float_as_uint(b);
+    }
+}
+
+// Norms of the right-hand side (the first residual, x = 0): r = d on active
+// cells.  Blocks cover (x, y) tiles and a run of `zper` planes.
+__global__ __launch_bounds__(BX * BY) void k_ref_norm0(Grid g, const float* __restrict__ d,
+                                                       const uint8_t* __restrict__ flags,
+                                                       double* __restrict__ part,
+                                                       unsigned* __restrict__ pmax, int zper)
+{
+    const int x = blockIdx.x * BX + threadIdx.x + 1;
+    const int y = blockIdx.y * BY + threadIdx.y + 1;
+    const int zb = blockIdx.z * zper + 1, ze = min(zb + zper, g.nz - 1);
+    double s2 = 0.0;
+    float m = 0.0f;
+    if (x <= g.nx - 2 && y <= g.ny - 2)
+        for (int z = zb; z < ze; ++z) {
+            const int i = g.idx(x, y, z);
+            if (flags[i] & F_ACTIVE) {
+                const float r = d[i];
+                s2 += (double)r * (double)r;
+                m = fmaxf(m, fabsf(r));
+            }
+        }
+    block_partials(s2, m, part, pmax);
+}
+
+// x_new = x_old + e/s on every interior cell (x_old = 0 when XZERO) and the
+// next negated residual r = (d + acc*x_new) - sum_n x_new_n on active cells,
+// with its partial norms; LAST skips the r store.  Neighbour values are
+// recomputed with the same two operations, so they equal the neighbours' own
+// x_new.  Once `done` is set the pass only copies x_old.
+template <bool XZERO, bool LAST>
+__global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __restrict__ xo,
+                                                       float* __restrict__ xn,
+                                                       const float* __restrict__ d,
+                                                       const uint8_t* __restrict__ flags,
+                                                       const uint16_t* __restrict__ e,
+                                                       float* __restrict__ q,
+                                                       const RefineCtl* __restrict__ ctl,
+                                                       double* __restrict__ part,
+                                                       unsigned* __restrict__ pmax, int zper)
+{
+    const int done = ctl->done;
+    const float sinv = ctl->sinv;
+    const int x = blockIdx.x * BX + threadIdx.x + 1;
+    const int y = blockIdx.y * BY + threadIdx.y + 1;
+    const int zb = blockIdx.z * zper + 1, ze = min(zb + zper, g.nz - 1);
+    double s2 = 0.0;
+    float m = 0.0f;
+    if (x <= g.nx - 2 && y <= g.ny - 2) {
+        for (int z = zb; z < ze; ++z) {
+            const int i = g.idx(x, y, z);
+            if (done) {
+                xn[i] = XZERO ? 0.0f : xo[i];
+                continue;
+            }
+            auto X = [&](int j) { return (XZERO ? 0.0f : xo[j]) + h2f(e[j]) * sinv; };
+            const float xc = X(i);
+            xn[i] = xc;
+            const uint8_t fl = flags[i];
+            float r = 0.0f;
+            if (fl & F_ACTIVE) {
+                const float sum = dev::sum6(X(i - 1), X(i + 1), X(i - g.px), X(i + g.px),
+                                            X(i - g.plane), X(i + g.plane));
+                r = (d[i] + (float)__popc(fl & F_NBRS) * xc) - sum;
+                s2 += (double)r * (double)r;
+                m = fmaxf(m, fabsf(r));
+            }
+            if (!LAST) q[i] = r;
+        }
+    }
+    block_partials(s2, m, part, pmax);
+}
+
+// Fixed-order reduction of the partials and the control update (one block of
+// 1024 threads).  stage 0: norms of d, a new step (resets done/refinements);
+// stage > 0: the residual after a refinement.
+__global__ __launch_bounds__(1024) void k_ref_finish(RefineCtl* __restrict__ ctl,
+                                                     const double* __restrict__ part,
+                                                     const unsigned* __restrict__ pmax, int n,
+                                                     int stage, double tol)
+{
+    __shared__ double ss[1024];
+    __shared__ unsigned sm[1024];
+    const int t = (int)threadIdx.x;
+    if (stage > 0 && ctl->done) return;  // uniform: keep the converged state
+    double a = 0.0;
+    unsigned b = 0;
+    for (int k = t; k < n; k += 1024) {
+        a += part[k];
+        b = max(b, pmax[k]);
+    }
+    ss[t] = a;
+    sm[t] = b;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] += ss[t + w];
+            sm[t] = max(sm[t], sm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double r2 = ss[0];
+    const unsigned qmax = sm[0];
+    if (stage == 0) {
+        ctl->bnorm2 = r2;
+        ctl->refinements = 0;
+        ctl->done = 0;
+    } else {
+        ctl->refinements = stage;
+    }
+    ctl->rnorm2 = r2;
+    const double bn = ctl->bnorm2;
+    ctl->rel = bn > 0.0 ? sqrt(r2 / bn) : 0.0;
+    ctl->qmax = qmax;
+    // s = 2^k with max|r| * s in [1/2, 1): k = 126 - biased exponent.
+    int k = 0;
+    if (qmax != 0) {
+        k = 126 - (int)(qmax >> 23);
+        k = k < -126 ? -126 : (k > 126 ? 126 : k);
+    }
+    ctl->s = __uint_as_float((unsigned)(k + 127) << 23);
+    ctl->sinv = __uint_as_float((unsigned)(127 - k) << 23);
+    if (qmax == 0 || (tol > 0.0 && ctl->rel <= tol)) ctl->done = 1;
+}
+
+// LDS of one FP16 sweep block: per level, two planes of the grown tile of
+// TXP x TY cell pairs, padded by one pair on each side.
+template <int TXP, int TY, int T>
+struct HalfLds {
+    uint32_t v[T][2][TY + 2][TXP + 2];
+};
+
+// T fused FP16 Jacobi sweeps on the correction for one tile, temporal blocking
+// as in the FP32 kernel (step_device.h: jacobi_fused_tile), with each thread
+// owning a pair of adjacent columns (x0, x0+1), x0 even.  The grown tile is
+// 2*TXP cells wide; the inner 2*TXP - 2T columns and TY - 2T rows are written
+// (T even, so the inner region is whole pairs).  Pairs cover x in [0, px): the
+// cells beyond the interior have inv = 0 and r = 0, so they stay e = 0.  FIRST:
+// e starts at 0 and the scaled right-hand side inv*r*s is formed from the FP32
+// r and written for the later launches of the refinement.
+template <int TXP, int TY, int T, bool FIRST>
+__global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t* __restrict__ ein,
+                                                         uint32_t* __restrict__ eout,
+                                                         const float* __restrict__ q,
+                                                         uint32_t* __restrict__ qi,
+                                                         const uint32_t* __restrict__ inv,
+                                                         const RefineCtl* __restrict__ ctl, int zc)
+{
+    static_assert(T % 2 == 0, "pairs: the halo must be whole pairs");
+    if (ctl->done) return;  // uniform across the grid
+    __shared__ HalfLds<TXP, TY, T> sh;
+    {
+        uint32_t* flat = &sh.v[0][0][0][0];
+        const int n = T * 2 * (TY + 2) * (TXP + 2);
+        for (int k = (int)(threadIdx.x + TXP * threadIdx.y); k < n; k += TXP * TY) flat[k] = 0u;
+    }
+    __syncthreads();
+    // XCD-contiguous tile order, as in the FP32 fused kernel.
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int nb = nbx * nby * (int)gridDim.z;
+    const int b = (int)blockIdx.x + nbx * ((int)blockIdx.y + nby * (int)blockIdx.z);
+    const int full = nb & ~7;
+    const int L = (b < full) ? (b & 7) * (full >> 3) + (b >> 3) : b;
+    const int tile_x = L % nbx, tile_y = (L / nbx) % nby, tile_z = L / (nbx * nby);
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    constexpr int OXC = 2 * TXP - 2 * T, OY = TY - 2 * T;
+    const int x0 = tile_x * OXC + 2 * tx - T;
+    const int y = tile_y * OY + ty - T + 1;
+    const int z0 = tile_z * zc + 1;
+    const int z1 = min(z0 + zc, g.nz - 1);
+    const bool in_grid = x0 >= 0 && x0 < g.px && y >= 0 && y <= g.ny;
+    const bool writes = tx >= T / 2 && tx < TXP - T / 2 && ty >= T && ty < TY - T && x0 >= 0 &&
+                        x0 < g.px && y >= 1 && y <= g.ny - 2;
+    const int wbase = in_grid ? (g.idx(x0, y, 0) >> 1) : 0;
+    const int wplane = g.plane >> 1;
+    const float s = FIRST ? ctl->s : 0.0f;
+
+    const h2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
+    h2 w[T][3];
+    h2 iq[T + 1], qq[T + 1];
+#pragma unroll
+    for (int t = 0; t < T; ++t) w[t][0] = w[t][1] = w[t][2] = zero2;
+#pragma unroll
+    for (int t = 0; t <= T; ++t) iq[t] = qq[t] = zero2;
+
+    h2 ne = zero2, ni = zero2, nq = zero2;
+    auto load = [&](int k) {
+        ne = zero2;
+        ni = zero2;
+        nq = zero2;
+        if (!(in_grid && k >= 0 && k <= g.nz)) return;
+        const int wi = wbase + k * wplane;
+        ni = ld_h2(inv, wi);
+        if (FIRST) {
+            const float2 rq = *reinterpret_cast<const float2*>(q + 2 * wi);
+            nq.x = (_Float16)(rq.x * s * (float)ni.x);
+            nq.y = (_Float16)(rq.y * s * (float)ni.y);
+            if (writes && k >= z0 && k < z1) st_h2(qi, wi, nq);
+        } else {
+            ne = ld_h2(ein, wi);
+            nq = ld_h2(qi, wi);
+        }
+    };
+    load(z0 - T);
+    for (int k = z0 - T; k < z1 + T; ++k) {
+        const h2 p0 = ne, ii = ni, qv = nq;
+        if (k + 1 < z1 + T) load(k + 1);
+#pragma unroll
+        for (int t = T; t > 0; --t) {
+            iq[t] = iq[t - 1];
+            qq[t] = qq[t - 1];
+        }
+        iq[0] = ii;
+        qq[0] = qv;
+        w[0][0] = w[0][1];
+        w[0][1] = w[0][2];
+        w[0][2] = p0;
+        const int cur = k & 1, prev = cur ^ 1;
+        h2 newv[T + 1];
+        newv[0] = p0;
+#pragma unroll
+        for (int t = 1; t <= T; ++t) {
+            const h2 Lp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 1][tx]);
+            const h2 Rp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 1][tx + 2]);
+            const h2 Up = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty][tx + 1]);
+            const h2 Dp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 2][tx + 1]);
+            const h2 C = w[t - 1][1];
+            const h2 xm = {Lp.y, C.x};
+            const h2 xp = {C.y, Rp.x};
+            const h2 sum = ((((xm + xp) + Up) + Dp) + w[t - 1][0]) + w[t - 1][2];
+            const h2 v = __builtin_elementwise_fma(iq[t], sum, -qq[t]);
+            newv[t] = v;
+            if (t < T) {
+                w[t][0] = w[t][1];
+                w[t][1] = w[t][2];
+                w[t][2] = v;
+            }
+        }
+        const int zo = k - T;
+        if (writes && zo >= z0 && zo < z1) st_h2(eout, wbase + zo * wplane, newv[T]);
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            sh.v[t][cur][ty + 1][tx + 1] = __builtin_bit_cast(uint32_t, newv[t]);
+        __syncthreads();
+    }
+}
+
+inline dim3 resid_grid(const Grid& g, int* zper)
+{
+    const int gx = (g.nx - 2 + BX - 1) / BX, gy = (g.ny - 2 + BY - 1) / BY;
+    int zs = kRefineParts / (gx * gy);
+    zs = zs < 1 ? 1 : (zs > g.nz - 2 ? g.nz - 2 : zs);
+    *zper = (g.nz - 2 + zs - 1) / zs;
+    zs = (g.nz - 2 + *zper - 1) / *zper;
+    return dim3((unsigned)gx, (unsigned)gy, (unsigned)zs);
+}
+
+template <int T, bool FIRST>
+hipError_t launch_sweeps(const Grid& g, const RefineBufs& b, const float* q, int ein, int eout,
+                         int zc, hipStream_t st)
+{
+    constexpr int TXP = 32, TY = 32;
+    constexpr int OXC = 2 * TXP - 2 * T, OY = TY - 2 * T;
+    dim3 grid((unsigned)((g.px + OXC - 1) / OXC), (unsigned)((g.ny - 2 + OY - 1) / OY),
+              (unsigned)((g.nz - 2 + zc - 1) / zc));
+    hipLaunchKernelGGL((k_ref_sweeps<TXP, TY, T, FIRST>), grid, dim3(TXP, TY), 0, st, g,
+                       (const uint32_t*)b.e[ein], b.e[eout], q, b.qi, (const uint32_t*)b.inv, b.ctl,
+                       zc);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t refine_build_inv(const Grid& g, const uint8_t* flags, const RefineBufs& b,
+                            hipStream_t st)
+{
+    dim3 grid((unsigned)((g.px + BX - 1) / BX), (unsigned)((g.ny + 1 + BY - 1) / BY),
+              (unsigned)(g.nz + 1));
+    hipLaunchKernelGGL(k_ref_inv, grid, dim3(BX, BY), 0, st, g, flags,
+                       reinterpret_cast<uint16_t*>(b.inv));
+    return hipGetLastError();
+}
+
+int refine_parts(const Grid& g)
+{
+    int zper = 0;
+    dim3 gr = resid_grid(g, &zper);
+    return (int)(gr.x * gr.y * gr.z);
+}
+
+hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, const RefineBufs& b,
+                        double tol, hipStream_t st)
+{
+    int zper = 0;
+    const dim3 grid = resid_grid(g, &zper);
+    hipLaunchKernelGGL(k_ref_norm0, grid, dim3(BX, BY), 0, st, g, d, flags, b.part, b.pmax, zper);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(1024), 0, st, b.ctl, (const double*)b.part,
+                       (const unsigned*)b.pmax, (int)(grid.x * grid.y * grid.z), 0, tol);
+    return hipGetLastError();
+}
+
+hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, int first, int ein,
+                         int eout, int sweeps, int zc, hipStream_t st)
+{
+    if (zc < 1) return hipErrorInvalidValue;
+    switch (sweeps * 2 + (first ? 1 : 0)) {
+    case 4: return launch_sweeps<2, false>(g, b, q, ein, eout, zc, st);
+    case 5: return launch_sweeps<2, true>(g, b, q, ein, eout, zc, st);
+    case 8: return launch_sweeps<4, false>(g, b, q, ein, eout, zc, st);
+    case 9: return launch_sweeps<4, true>(g, b, q, ein, eout, zc, st);
+    case 12: return launch_sweeps<6, false>(g, b, q, ein, eout, zc, st);
+    case 13: return launch_sweeps<6, true>(g, b, q, ein, eout, zc, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t refine_resid(const Grid& g, const float* xo, float* xn, const float* d,
+                        const uint8_t* flags, const RefineBufs& b, int ein, int xzero, int last,
+                        int stage, double tol, hipStream_t st)
+{
+    int zper = 0;
+    const dim3 grid = resid_grid(g, &zper);
+    const uint16_t* e = reinterpret_cast<const uint16_t*>(b.e[ein]);
+    if (xzero && last)
+        hipLaunchKernelGGL((k_ref_resid<true, true>), grid, dim3(BX, BY), 0, st, g, xo, xn, d, flags,
+                           e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+    else if (xzero)
+        hipLaunchKernelGGL((k_ref_resid<true, false>), grid, dim3(BX, BY), 0, st, g, xo, xn, d,
+                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+    else if (last)
+        hipLaunchKernelGGL((k_ref_resid<false, true>), grid, dim3(BX, BY), 0, st, g, xo, xn, d,
+                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+    else
+        hipLaunchKernelGGL((k_ref_resid<false, false>), grid, dim3(BX, BY), 0, st, g, xo, xn, d,
+                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+    hipError_t e2 = hipGetLastError();
+    if (e2 != hipSuccess) return e2;
+    hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(1024), 0, st, b.ctl, (const double*)b.part,
+                       (const unsigned*)b.pmax, (int)(grid.x * grid.y * grid.z), stage, tol);
+    return hipGetLastError();
+}
+
+}  // namespace smk

@@ -58,15 +58,30 @@ Grid make_grid(unsigned nx, unsigned ny, unsigned nz)
     return g;
 }
 
-constexpr int kFloatFields = 11;  // rho0 rho1 u0 u1 v0 v1 w0 w1 P0 P1 d
+constexpr int kStoredFields = 8;  // rho0 rho1 u0 u1 v0 v1 w0 w1 (FP32 or BF16)
+constexpr int kFloatFields = 3;   // P0 P1 d (always FP32)
 constexpr int kByteFields = 3;    // base s flags
 
-size_t arena_bytes(const Grid& g)
+size_t stored_bytes(int storage) { return storage == SMOKE_STORAGE_BF16 ? 2 : 4; }
+
+// Arena: stored fields, P0 P1 d, staging (reference-layout faces, FP32), then
+// the byte fields.
+size_t arena_bytes(const Grid& g, int storage)
 {
     size_t e = (size_t)g.elems;
     size_t staging = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) * sizeof(float);
-    return kFloatFields * align256(e * sizeof(float)) + align256(staging) +
+    return kStoredFields * align256(e * stored_bytes(storage)) +
+           kFloatFields * align256(e * sizeof(float)) + align256(staging) +
            kByteFields * align256(e);
+}
+
+// REFINED buffers: q (FP32), e0 e1 qi inv (FP16), control block, partials.
+size_t refine_bytes(const Grid& g)
+{
+    size_t e = (size_t)g.elems;
+    size_t parts = (size_t)smk::refine_parts(g);
+    return align256(e * sizeof(float)) + 4 * align256(e * 2) + align256(sizeof(smk::RefineCtl)) +
+           align256(parts * sizeof(double)) + align256(parts * sizeof(unsigned));
 }
 
 bool dims_ok(unsigned nx, unsigned ny, unsigned nz)
@@ -80,8 +95,19 @@ bool dims_ok(unsigned nx, unsigned ny, unsigned nz)
 smoke_status check_params(const smoke_params* p)
 {
     if (!p) return fail(SMOKE_ERR_INVALID, "params: NULL");
-    if (p->projection < SMOKE_PROJ_REFERENCE || p->projection > SMOKE_PROJ_JACOBI)
+    if (p->projection < SMOKE_PROJ_REFERENCE || p->projection > SMOKE_PROJ_REFINED)
         return fail(SMOKE_ERR_INVALID, "params: unknown projection %d", p->projection);
+    if (p->projection == SMOKE_PROJ_REFINED) {
+        if (p->inner_sweeps < 2 || p->inner_sweeps > 1000 || (p->inner_sweeps & 1))
+            return fail(SMOKE_ERR_INVALID, "params: inner_sweeps %d not an even number in 2..1000",
+                        p->inner_sweeps);
+        if (!std::isfinite(p->tolerance) || p->tolerance < 0.0)
+            return fail(SMOKE_ERR_INVALID, "params: tolerance must be finite and >= 0");
+        if (p->warm_start)
+            return fail(SMOKE_ERR_INVALID, "params: the REFINED projection starts cold (warm_start 0)");
+        if (p->iterations < 1)
+            return fail(SMOKE_ERR_INVALID, "params: REFINED needs iterations >= 1");
+    }
     if (p->obstacle_mode != SMOKE_OBSTACLES_LAST_WINS && p->obstacle_mode != SMOKE_OBSTACLES_UNION)
         return fail(SMOKE_ERR_INVALID, "params: unknown obstacle_mode %d", p->obstacle_mode);
     if (p->iterations < 0 || p->iterations > 1000000)
@@ -131,10 +157,55 @@ hipError_t smk::refresh_mask(smoke_sim* s, long long* launches)
     if (e != hipSuccess) return e;
     e = smk::build_flags(s->g, s->s, s->flags, s->stream);
     if (e != hipSuccess) return e;
-    s->mask_dirty = false;
     if (launches) *launches += 2;
+    if (s->refine_mem) {
+        e = smk::refine_build_inv(s->g, s->flags, s->rb, s->stream);
+        if (e != hipSuccess) return e;
+        if (launches) *launches += 1;
+    }
+    s->mask_dirty = false;
     return hipSuccess;
 }
+
+namespace {
+
+// Allocates and clears the REFINED buffers (once per simulation).
+smoke_status ensure_refine(smoke_sim* s)
+{
+    if (s->refine_mem) return SMOKE_OK;
+    const Grid& g = s->g;
+    const size_t bytes = refine_bytes(g);
+    char* m = nullptr;
+    if (hipMalloc((void**)&m, bytes) != hipSuccess)
+        return fail(SMOKE_ERR_NOMEM, "refined solver: cannot allocate %.2f GiB", bytes / 1073741824.0);
+    const size_t e = (size_t)g.elems;
+    char* p = m;
+    smk::RefineBufs& b = s->rb;
+    b.q = reinterpret_cast<float*>(p);
+    p += align256(e * sizeof(float));
+    b.e[0] = reinterpret_cast<uint32_t*>(p);
+    p += align256(e * 2);
+    b.e[1] = reinterpret_cast<uint32_t*>(p);
+    p += align256(e * 2);
+    b.qi = reinterpret_cast<uint32_t*>(p);
+    p += align256(e * 2);
+    b.inv = reinterpret_cast<uint32_t*>(p);
+    p += align256(e * 2);
+    b.ctl = reinterpret_cast<smk::RefineCtl*>(p);
+    p += align256(sizeof(smk::RefineCtl));
+    b.part = reinterpret_cast<double*>(p);
+    p += align256((size_t)smk::refine_parts(g) * sizeof(double));
+    b.pmax = reinterpret_cast<unsigned*>(p);
+    s->refine_mem = m;
+    hipError_t err = smk::fill_u8(reinterpret_cast<uint8_t*>(m), (long long)bytes, 0, s->stream);
+    s->mask_dirty = true;  // builds inv with the flags
+    if (err == hipSuccess) err = smk::refresh_mask(s, nullptr);
+    if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
+    if (err != hipSuccess) return fail(SMOKE_ERR_HIP, "refined solver: init failed: %s", hipGetErrorString(err));
+    return SMOKE_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -146,9 +217,11 @@ hipError_t stage_scene(smoke_sim* s, long long* launches)
     for (const Object& o : s->objects) {
         if (o.type != 1) continue;
         int launched = 0;
-        hipError_t e = smk::fill_source(s->g, s->rho[0], s->rho[1],
-                                        make_float4(o.pos[0], o.pos[1], o.pos[2], o.r), s->stream,
-                                        &launched);
+        hipError_t e = smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+            return smk::fill_source(s->g, rho[0], rho[1],
+                                    make_float4(o.pos[0], o.pos[1], o.pos[2], o.r), s->stream,
+                                    &launched);
+        });
         if (e != hipSuccess) return e;
         *launches += launched;
     }
@@ -166,7 +239,49 @@ hipError_t stage_force(smoke_sim* s, float dt, long long* launches)
     a.gate_on = s->prm.density_gate > 0.0f;
     const int n = s->now;
     *launches += 1;
-    return smk::force(s->g, s->u[n], s->v[n], s->w[n], s->rho[n], s->flags, a, s->stream);
+    return smk::with_fields(s, [&](auto rho, auto u, auto v, auto w) {
+        return smk::force(s->g, u[n], v[n], w[n], rho[n], s->flags, a, s->stream);
+    });
+}
+
+// REFINED: R refinements of inner_sweeps FP16 sweeps (launches of an even
+// number of fused sweeps), each closed by the FP32 update/residual pass; the
+// result ends in P[pcur].
+hipError_t enqueue_refined(smoke_sim* s, long long* launches, long long* sweep_launches)
+{
+    const Grid& g = s->g;
+    const smoke_params& p = s->prm;
+    const smk::RefineBufs& b = s->rb;
+    const int R = (p.iterations + p.inner_sweeps - 1) / p.inner_sweeps;
+    int T = s->plan.sweeps_per_launch;
+    if (T < 2 || T > 6 || (T & 1)) T = 4;
+    hipError_t e = smk::refine_norm0(g, s->d, s->flags, b, p.tolerance, s->stream);
+    if (e != hipSuccess) return e;
+    *launches += 2;
+    int a = s->pcur;
+    for (int r = 0; r < R; ++r) {
+        int ein = 0;
+        bool first = true;
+        for (int left = p.inner_sweeps; left > 0;) {
+            const int t = std::min(left, T);
+            const int eout = first ? 0 : 1 - ein;
+            e = smk::refine_sweeps(g, b, r == 0 ? s->d : b.q, first ? 1 : 0, ein, eout, t,
+                                   s->plan.zc, s->stream);
+            if (e != hipSuccess) return e;
+            ein = eout;
+            first = false;
+            left -= t;
+            *launches += 1;
+            *sweep_launches += 1;
+        }
+        e = smk::refine_resid(g, s->P[a], s->P[a ^ 1], s->d, s->flags, b, ein, r == 0 ? 1 : 0,
+                              r == R - 1 ? 1 : 0, r + 1, p.tolerance, s->stream);
+        if (e != hipSuccess) return e;
+        *launches += 2;
+        a ^= 1;
+    }
+    s->pcur = a;
+    return hipSuccess;
 }
 
 // K Jacobi sweeps as launches of plan.sweeps_per_launch fused sweeps each, the
@@ -215,10 +330,17 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
             }
         return sweep_mark(s, 1, 2LL * p.iterations);
     }
-    e = smk::divergence(g, s->u[n], s->v[n], s->w[n], s->flags, s->d, s->stream);
+    e = smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+        return smk::divergence(g, u[n], v[n], w[n], s->flags, s->d, s->stream);
+    });
     if (e != hipSuccess) return e;
     *launches += 1;
-    if (p.projection == SMOKE_PROJ_JACOBI) {
+    if (p.projection == SMOKE_PROJ_REFINED) {
+        if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
+        long long sweep_launches = 0;
+        if ((e = enqueue_refined(s, launches, &sweep_launches)) != hipSuccess) return e;
+        if ((e = sweep_mark(s, 1, sweep_launches)) != hipSuccess) return e;
+    } else if (p.projection == SMOKE_PROJ_JACOBI) {
         const bool zero_in = !p.warm_start;
         if (p.iterations == 0 && zero_in) {  // P = 0: no correction
             if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
@@ -249,7 +371,9 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
             }
         if ((e = sweep_mark(s, 1, 2LL * p.iterations)) != hipSuccess) return e;
     }
-    e = smk::gradient(g, s->u[n], s->v[n], s->w[n], s->P[s->pcur], s->flags, s->stream);
+    e = smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+        return smk::gradient(g, u[n], v[n], w[n], s->P[s->pcur], s->flags, s->stream);
+    });
     *launches += 1;
     return e;
 }
@@ -258,16 +382,19 @@ hipError_t stage_advect_velocity(smoke_sim* s, float dt, long long* launches)
 {
     const int n = s->now, p = 1 - s->now;
     *launches += 1;
-    return smk::advect_velocity(s->g, s->u[n], s->v[n], s->w[n], s->u[p], s->v[p], s->w[p], s->flags,
-                                dt, s->stream);
+    return smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+        return smk::advect_velocity(s->g, u[n], v[n], w[n], u[p], v[p], w[p], s->flags, dt,
+                                    s->stream);
+    });
 }
 
 hipError_t stage_advect_density(smoke_sim* s, float dt, long long* launches)
 {
     const int n = s->now, p = 1 - s->now;
     *launches += 1;
-    return smk::advect_density(s->g, s->rho[n], s->rho[p], s->u[p], s->v[p], s->w[p], s->flags, dt,
-                               s->stream);
+    return smk::with_fields(s, [&](auto rho, auto u, auto v, auto w) {
+        return smk::advect_density(s->g, rho[n], rho[p], u[p], v[p], w[p], s->flags, dt, s->stream);
+    });
 }
 
 hipError_t run_stage(smoke_sim* s, int stage, float dt, long long* launches)
@@ -362,7 +489,8 @@ smoke_status step_locked(smoke_sim* s, float dt)
 
 // Blocking device->host copy of a padded cell field through the staging
 // buffer.
-smoke_status read_cells_f32(smoke_sim* s, const float* src, float* dst)
+template <typename T>
+smoke_status read_cells_f32(smoke_sim* s, const T* src, float* dst)
 {
     SMOKE_WAIT_BATCH(s);
     HIP_TRY(smk::pack_cells_f32(s->g, src, s->staging, s->stream));
@@ -380,7 +508,7 @@ extern "C" {
 
 const char* smoke_last_error(void) { return g_last_error.c_str(); }
 
-const char* smoke_version(void) { return "smokesim 0.1 (gfx950, FP32)"; }
+const char* smoke_version(void) { return "smokesim 0.2 (gfx950, FP32 and BF16/FP16 mixed)"; }
 
 smoke_status smoke_device_info(char* buf, size_t len)
 {
@@ -411,26 +539,49 @@ smoke_status smoke_params_default(smoke_params* out)
     out->iterations = 30;
     out->obstacle_mode = SMOKE_OBSTACLES_LAST_WINS;
     out->warm_start = 0;
+    out->inner_sweeps = 20;
+    out->reserved = 0;
+    out->tolerance = 0.0;
     return SMOKE_OK;
 }
 
 size_t smoke_footprint_bytes(unsigned nx, unsigned ny, unsigned nz)
 {
-    if (!dims_ok(nx, ny, nz)) return 0;
-    return arena_bytes(make_grid(nx, ny, nz));
+    return smoke_footprint_bytes_ex(nx, ny, nz, SMOKE_STORAGE_FP32, 0);
 }
+
+size_t smoke_footprint_bytes_ex(unsigned nx, unsigned ny, unsigned nz, int storage, int refined)
+{
+    if (!dims_ok(nx, ny, nz) || (storage != SMOKE_STORAGE_FP32 && storage != SMOKE_STORAGE_BF16))
+        return 0;
+    const Grid g = make_grid(nx, ny, nz);
+    return arena_bytes(g, storage) + (refined ? refine_bytes(g) : 0);
+}
+
+int smoke_get_storage(const smoke_sim* s) { return s ? s->storage : -1; }
 
 smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* density,
                           smoke_sim** out)
+{
+    return smoke_create_ex(nx, ny, nz, density, SMOKE_STORAGE_FP32, out);
+}
+
+smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float* density,
+                             int storage, smoke_sim** out)
 {
     if (!out) return fail(SMOKE_ERR_INVALID, "create: out is NULL");
     *out = nullptr;
     if (!dims_ok(nx, ny, nz))
         return fail(SMOKE_ERR_INVALID, "create: dims %ux%ux%u invalid (each >= 3, padded size < 2^31)",
                     nx, ny, nz);
+    if (storage != SMOKE_STORAGE_FP32 && storage != SMOKE_STORAGE_BF16)
+        return fail(SMOKE_ERR_INVALID, "create: unknown storage %d", storage);
     smoke_sim* s = new smoke_sim();
     s->g = make_grid(nx, ny, nz);
+    s->storage = storage;
     smoke_params_default(&s->prm);
+    // BF16 fields cannot run the face-based reference projection.
+    if (storage == SMOKE_STORAGE_BF16) s->prm.projection = SMOKE_PROJ_JACOBI;
     auto bail = [&](smoke_status st) {
         smoke_destroy(s);
         return st;
@@ -439,17 +590,28 @@ smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* de
     if (e != hipSuccess) return bail(fail(SMOKE_ERR_HIP, "create: no device: %s", hipGetErrorString(e)));
     e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail(SMOKE_ERR_HIP, "create: stream: %s", hipGetErrorString(e)));
-    size_t bytes = arena_bytes(s->g);
+    size_t bytes = arena_bytes(s->g, storage);
     e = hipMalloc((void**)&s->arena, bytes);
     if (e != hipSuccess) {
         s->arena = nullptr;
         return bail(fail(SMOKE_ERR_NOMEM, "create: cannot allocate %.2f GiB: %s", bytes / 1073741824.0,
                          hipGetErrorString(e)));
     }
+    const size_t sbytes = align256((size_t)s->g.elems * stored_bytes(storage));
     const size_t fbytes = align256((size_t)s->g.elems * sizeof(float));
     char* p = s->arena;
-    float** fields[kFloatFields] = {&s->rho[0], &s->rho[1], &s->u[0], &s->u[1], &s->v[0], &s->v[1],
-                                    &s->w[0],   &s->w[1],   &s->P[0], &s->P[1], &s->d};
+    for (int k = 0; k < kStoredFields; ++k) {
+        const int h = k & 1;
+        if (storage == SMOKE_STORAGE_BF16) {
+            smk::bf16s** f[4] = {s->hrho, s->hu, s->hv, s->hw};
+            f[k >> 1][h] = reinterpret_cast<smk::bf16s*>(p);
+        } else {
+            float** f[4] = {s->rho, s->u, s->v, s->w};
+            f[k >> 1][h] = reinterpret_cast<float*>(p);
+        }
+        p += sbytes;
+    }
+    float** fields[kFloatFields] = {&s->P[0], &s->P[1], &s->d};
     for (int k = 0; k < kFloatFields; ++k) {
         *fields[k] = reinterpret_cast<float*>(p);
         p += fbytes;
@@ -462,14 +624,17 @@ smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* de
         p += align256((size_t)s->g.elems);
     }
     // Zero everything (the reference leaves half [1] uninitialised; zero is
-    // the defined value here), then the floor mask and the initial density.
-    e = smk::fill_f32(s->rho[0], (long long)(kFloatFields * fbytes / sizeof(float)), 0.0f, s->stream);
-    if (e == hipSuccess) e = smk::fill_u8(s->base, (long long)(kByteFields * align256(s->g.elems)), 0, s->stream);
+    // the defined value here; all-zero bits are +0 in FP32 and BF16), then
+    // the floor mask and the initial density.
+    e = smk::fill_u8(reinterpret_cast<uint8_t*>(s->arena), (long long)bytes, 0, s->stream);
     if (e == hipSuccess) e = smk::floor_mask(s->g, s->base, s->stream);
     if (e == hipSuccess && density) {
         size_t db = (size_t)nx * ny * nz * sizeof(float);
         e = hipMemcpyAsync(s->staging, density, db, hipMemcpyHostToDevice, s->stream);
-        if (e == hipSuccess) e = smk::unpack_cells_f32(s->g, s->staging, s->rho[0], s->stream);
+        if (e == hipSuccess)
+            e = smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+                return smk::unpack_cells_f32(s->g, s->staging, rho[0], s->stream);
+            });
     }
     if (e == hipSuccess) e = refresh_mask(s, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -498,6 +663,7 @@ void smoke_destroy(smoke_sim* s)
         if (h) (void)hipHostFree(h);
     if (s->rb_dev) (void)hipFree(s->rb_dev);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    if (s->refine_mem) (void)hipFree(s->refine_mem);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -534,7 +700,7 @@ smoke_status smoke_readback_begin(smoke_sim* s, int format)
     }
     const int slot = s->rb_next;
     char* dev = static_cast<char*>(s->rb_dev) + slot * slot_bytes;
-    const float* src = s->rho[1 - s->now];
+    const int src_half = 1 - s->now;
     // The copy stream packs after the last enqueued step; the next step's
     // scene stage (which writes sources into both density halves) waits for
     // the pack, not for the host copy.  A simulation in a batch is stepped on
@@ -544,9 +710,15 @@ smoke_status smoke_readback_begin(smoke_sim* s, int format)
     HIP_TRY(hipStreamWaitEvent(s->copy_stream, s->rb_step_done, 0));
     size_t bytes = slot_bytes;
     if (format == SMOKE_READBACK_F32) {
-        HIP_TRY(smk::pack_cells_f32(s->g, src, reinterpret_cast<float*>(dev), s->copy_stream));
+        HIP_TRY(smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+            return smk::pack_cells_f32(s->g, rho[src_half], reinterpret_cast<float*>(dev),
+                                       s->copy_stream);
+        }));
     } else {
-        HIP_TRY(smk::pack_cells_half(s->g, src, reinterpret_cast<uint16_t*>(dev), format, s->copy_stream));
+        HIP_TRY(smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+            return smk::pack_cells_half(s->g, rho[src_half], reinterpret_cast<uint16_t*>(dev), format,
+                                        s->copy_stream);
+        }));
         bytes = cells * sizeof(uint16_t);
     }
     HIP_TRY(hipEventRecord(s->rb_packed[slot], s->copy_stream));
@@ -587,8 +759,28 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
     if (st != SMOKE_OK) return st;
     std::lock_guard<std::mutex> lk(s->mu);
     SMOKE_REFUSE_IN_BATCH(s, "set_params");
+    if (s->storage == SMOKE_STORAGE_BF16 && p->projection == SMOKE_PROJ_REFERENCE)
+        return fail(SMOKE_ERR_INVALID,
+                    "set_params: BF16 storage runs the pressure forms (RBSOR, JACOBI, REFINED), not REFERENCE");
     if (p->obstacle_mode != s->prm.obstacle_mode) s->mask_dirty = true;
+    if (p->projection == SMOKE_PROJ_REFINED) {
+        smoke_status rs = ensure_refine(s);
+        if (rs != SMOKE_OK) return rs;
+    }
     s->prm = *p;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_solve_info(smoke_sim* s, double* rel_residual, int* refinements)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "get_solve_info: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->refine_mem) return fail(SMOKE_ERR_STATE, "get_solve_info: the REFINED solver never ran");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    smk::RefineCtl c;
+    HIP_TRY(hipMemcpy(&c, s->rb.ctl, sizeof c, hipMemcpyDeviceToHost));
+    if (rel_residual) *rel_residual = c.rel;
+    if (refinements) *refinements = c.refinements;
     return SMOKE_OK;
 }
 
@@ -750,7 +942,9 @@ smoke_status smoke_set_density(smoke_sim* s, int half, const float* src)
     SMOKE_REFUSE_IN_BATCH(s, "set_density");
     size_t bytes = (size_t)s->g.nx * s->g.ny * s->g.nz * sizeof(float);
     HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(smk::unpack_cells_f32(s->g, s->staging, s->rho[half], s->stream));
+    HIP_TRY(smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+        return smk::unpack_cells_f32(s->g, s->staging, rho[half], s->stream);
+    }));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SMOKE_OK;
 }
@@ -759,30 +953,22 @@ smoke_status smoke_get_density(smoke_sim* s, int half, float* dst)
 {
     if (!s || !dst || (half != 0 && half != 1)) return fail(SMOKE_ERR_INVALID, "get_density: bad argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    return read_cells_f32(s, s->rho[half], dst);
-}
-
-static float* comp_ptr(smoke_sim* s, int comp, int half)
-{
-    if (half != 0 && half != 1) return nullptr;
-    switch (comp) {
-    case 0: return s->u[half];
-    case 1: return s->v[half];
-    case 2: return s->w[half];
-    default: return nullptr;
-    }
+    return smk::with_fields(s, [&](auto rho, auto, auto, auto) { return read_cells_f32(s, rho[half], dst); });
 }
 
 smoke_status smoke_set_velocity(smoke_sim* s, int comp, int half, const float* src)
 {
     if (!s || !src) return fail(SMOKE_ERR_INVALID, "set_velocity: NULL");
-    float* dst = comp_ptr(s, comp, half);
-    if (!dst) return fail(SMOKE_ERR_INVALID, "set_velocity: bad component %d or half %d", comp, half);
+    if ((half != 0 && half != 1) || comp < 0 || comp > 2)
+        return fail(SMOKE_ERR_INVALID, "set_velocity: bad component %d or half %d", comp, half);
     std::lock_guard<std::mutex> lk(s->mu);
     SMOKE_REFUSE_IN_BATCH(s, "set_velocity");
     size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
     HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(smk::unpack_faces_f32(s->g, s->staging, dst, s->stream));
+    HIP_TRY(smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+        auto dst = comp == 0 ? u[half] : (comp == 1 ? v[half] : w[half]);
+        return smk::unpack_faces_f32(s->g, s->staging, dst, s->stream);
+    }));
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SMOKE_OK;
 }
@@ -790,11 +976,14 @@ smoke_status smoke_set_velocity(smoke_sim* s, int comp, int half, const float* s
 smoke_status smoke_get_velocity(smoke_sim* s, int comp, int half, float* dst)
 {
     if (!s || !dst) return fail(SMOKE_ERR_INVALID, "get_velocity: NULL");
-    float* src = comp_ptr(s, comp, half);
-    if (!src) return fail(SMOKE_ERR_INVALID, "get_velocity: bad component %d or half %d", comp, half);
+    if ((half != 0 && half != 1) || comp < 0 || comp > 2)
+        return fail(SMOKE_ERR_INVALID, "get_velocity: bad component %d or half %d", comp, half);
     std::lock_guard<std::mutex> lk(s->mu);
     SMOKE_WAIT_BATCH(s);
-    HIP_TRY(smk::pack_faces_f32(s->g, src, s->staging, s->stream));
+    HIP_TRY(smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+        auto src = comp == 0 ? u[half] : (comp == 1 ? v[half] : w[half]);
+        return smk::pack_faces_f32(s->g, src, s->staging, s->stream);
+    }));
     size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
     HIP_TRY(hipMemcpyAsync(dst, s->staging, bytes, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -847,7 +1036,10 @@ smoke_status smoke_step(smoke_sim* s, float dt, float* density_out)
     std::lock_guard<std::mutex> lk(s->mu);
     smoke_status st = step_locked(s, dt);
     if (st != SMOKE_OK) return st;
-    if (density_out) return read_cells_f32(s, s->rho[1 - s->now], density_out);
+    if (density_out)
+        return smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+            return read_cells_f32(s, rho[1 - s->now], density_out);
+        });
     return SMOKE_OK;
 }
 
@@ -868,7 +1060,9 @@ smoke_status smoke_read_density(smoke_sim* s, float* dst)
 {
     if (!s || !dst) return fail(SMOKE_ERR_INVALID, "read_density: NULL");
     std::lock_guard<std::mutex> lk(s->mu);
-    return read_cells_f32(s, s->rho[1 - s->now], dst);
+    return smk::with_fields(s, [&](auto rho, auto, auto, auto) {
+        return read_cells_f32(s, rho[1 - s->now], dst);
+    });
 }
 
 smoke_status smoke_synchronize(smoke_sim* s)

@@ -44,6 +44,17 @@ struct smoke_sim {
     float* w[2] = {nullptr, nullptr};
     float* P[2] = {nullptr, nullptr};
     float* d = nullptr;
+    // SMOKE_STORAGE_BF16: density and u, v, w live in these instead (the
+    // float pointers above are then NULL).
+    int storage = SMOKE_STORAGE_FP32;
+    smk::bf16s* hrho[2] = {nullptr, nullptr};
+    smk::bf16s* hu[2] = {nullptr, nullptr};
+    smk::bf16s* hv[2] = {nullptr, nullptr};
+    smk::bf16s* hw[2] = {nullptr, nullptr};
+    // REFINED solver buffers (allocated when the projection is first set to
+    // REFINED).
+    char* refine_mem = nullptr;
+    smk::RefineBufs rb;
     float* staging = nullptr;
     uint8_t* base = nullptr;
     uint8_t* s = nullptr;
@@ -94,6 +105,14 @@ struct smoke_sim {
 };
 
 namespace smk {
+// Calls fn(rho, u, v, w) with the simulation's field pointer pairs typed by
+// its storage (float* [2] or bf16s* [2]).
+template <typename Fn>
+auto with_fields(smoke_sim* s, Fn&& fn)
+{
+    if (s->storage == SMOKE_STORAGE_BF16) return fn(s->hrho, s->hu, s->hv, s->hw);
+    return fn(s->rho, s->u, s->v, s->w);
+}
 // Rebuilds mask and flags if the base mask or an obstacle changed.
 hipError_t refresh_mask(smoke_sim* s, long long* launches);
 // Called by smoke_destroy for a simulation that is still in a batch: waits

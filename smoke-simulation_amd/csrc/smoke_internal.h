@@ -32,6 +32,26 @@ struct Grid {
 constexpr int kBlockX = 64;
 constexpr int kBlockY = 4;
 
+// BF16 storage element of the mixed-precision form (the upper half of an
+// FP32 word; arithmetic stays FP32).
+struct bf16s {
+    uint16_t bits;
+};
+
+// Device-side state of the mixed-precision pressure solve of one simulation
+// (refine_kernels.hip): FP64 norms, the power-of-two scale of the current
+// correction and the convergence decision.
+struct RefineCtl {
+    double bnorm2;     // ||b||^2 of this step (b = divergence on active cells)
+    double rnorm2;     // ||r||^2 at the last residual pass
+    double rel;        // sqrt(rnorm2 / bnorm2)
+    float s, sinv;     // correction scale 2^k and its inverse
+    unsigned qmax;     // max |r| as FP32 bits (order-preserving for r >= 0)
+    int done;          // relative residual reached: later passes of the step idle
+    int refinements;   // refinements run this step
+    int pad;
+};
+
 // Cell flag bits (one byte per cell).
 enum : uint8_t {
     F_SELF = 1,
@@ -179,5 +199,58 @@ hipError_t advect_velocity(const Grid& g, const float* u0, const float* v0, cons
 hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float* u,
                           const float* v, const float* w, const uint8_t* flags, float dt,
                           hipStream_t st);
+
+// BF16-stored fields (FP32 arithmetic): the same stages and conversions.
+hipError_t unpack_cells_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st);
+hipError_t pack_cells_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st);
+hipError_t unpack_faces_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st);
+hipError_t pack_faces_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st);
+hipError_t pack_cells_half(const Grid& g, const bf16s* src, uint16_t* dst, int format,
+                           hipStream_t st);
+hipError_t fill_source(const Grid& g, bf16s* rho0, bf16s* rho1, float4 sph, hipStream_t st,
+                       int* launched);
+hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const bf16s* rho,
+                 const uint8_t* flags, const ForceArgs& a, hipStream_t st);
+hipError_t divergence(const Grid& g, const bf16s* u, const bf16s* v, const bf16s* w,
+                      const uint8_t* flags, float* d, hipStream_t st);
+hipError_t gradient(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* p,
+                    const uint8_t* flags, hipStream_t st);
+hipError_t advect_velocity(const Grid& g, const bf16s* u0, const bf16s* v0, const bf16s* w0,
+                           bf16s* u1, bf16s* v1, bf16s* w1, const uint8_t* flags, float dt,
+                           hipStream_t st);
+hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s* u,
+                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
+                          hipStream_t st);
+
+// ------------------------------------------- mixed-precision pressure solve
+// Device buffers of the refinement loop (refine_kernels.hip), padded layout.
+// e, qi and inv hold FP16 values, two cells per 32-bit word.
+struct RefineBufs {
+    float* q = nullptr;            // negated FP32 residual of the current refinement
+    uint32_t* e[2] = {nullptr, nullptr};  // FP16 correction, ping-pong
+    uint32_t* qi = nullptr;        // FP16 inv * r * s
+    uint32_t* inv = nullptr;       // FP16 diagonal inverse (0 off the active set)
+    RefineCtl* ctl = nullptr;
+    double* part = nullptr;        // per-block FP64 partial sums of r^2
+    unsigned* pmax = nullptr;      // per-block max |r| (FP32 bits)
+};
+// Target number of blocks of a residual pass (one partial each).
+constexpr int kRefineParts = 4096;
+// Blocks (partials) of a residual pass over g.
+int refine_parts(const Grid& g);
+hipError_t refine_build_inv(const Grid& g, const uint8_t* flags, const RefineBufs& b,
+                            hipStream_t st);
+// Norms of d (the residual of x = 0) and the step's control reset.
+hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, const RefineBufs& b,
+                        double tol, hipStream_t st);
+// `sweeps` (2, 4 or 6) FP16 sweeps e[ein] -> e[eout]; `first`: e starts at 0
+// and the scaled right-hand side is formed from the FP32 q.
+hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, int first, int ein,
+                         int eout, int sweeps, int zc, hipStream_t st);
+// x_new = x_old + e[ein]/s and the next residual (norms, control update for
+// refinement `stage`).
+hipError_t refine_resid(const Grid& g, const float* xo, float* xn, const float* d,
+                        const uint8_t* flags, const RefineBufs& b, int ein, int xzero, int last,
+                        int stage, double tol, hipStream_t st);
 
 }  // namespace smk

@@ -41,15 +41,36 @@ __device__ __forceinline__ bool is_interior(const Grid& g, int x, int y, int z)
     return x >= 1 && x <= g.nx - 2 && y >= 1 && y <= g.ny - 2 && z >= 1 && z <= g.nz - 2;
 }
 
+// Field storage: FP32, or BF16 (mixed-precision form) widened to FP32 on load
+// and rounded to nearest even on store.  The step bodies below are templates
+// over the element type; with float they are plain loads and stores.
+__device__ __forceinline__ float ldf(const float* __restrict__ p, int i) { return p[i]; }
+__device__ __forceinline__ float ldf(const bf16s* __restrict__ p, int i)
+{
+    return __uint_as_float((uint32_t)p[i].bits << 16);
+}
+__device__ __forceinline__ uint16_t bf16_rne(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x0040u);  // quiet NaN
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ void stf(float* __restrict__ p, int i, float v) { p[i] = v; }
+__device__ __forceinline__ void stf(bf16s* __restrict__ p, int i, float v)
+{
+    p[i].bits = bf16_rne(v);
+}
+
 // fillSmoke (smokeSimulation.cu:251-273) for one cell the caller knows to be
 // interior: density 1.0 in both halves if the cell is inside the source.
-__device__ __forceinline__ void source_at(const Grid& g, float* __restrict__ r0,
-                                          float* __restrict__ r1, float4 sph, int x, int y, int z)
+template <typename T>
+__device__ __forceinline__ void source_at(const Grid& g, T* __restrict__ r0, T* __restrict__ r1,
+                                          float4 sph, int x, int y, int z)
 {
     if (inside(sph, x, y, z)) {
         int i = g.idx(x, y, z);
-        r0[i] = 1.0f;
-        r1[i] = 1.0f;
+        stf(r0, i, 1.0f);
+        stf(r1, i, 1.0f);
     }
 }
 
@@ -57,19 +78,19 @@ __device__ __forceinline__ void source_at(const Grid& g, float* __restrict__ r0,
 // fused per staggered index: the buoyancy range is x in [0,nx-1],
 // y in [1,ny-1], z in [0,nz-1]; the clamp range x,y,z in [1,n-1] lies inside
 // it and sees the post-buoyancy v of the same index.
-__device__ __forceinline__ void force_at(const Grid& g, float* __restrict__ u,
-                                         float* __restrict__ v, float* __restrict__ w,
-                                         const float* __restrict__ rho,
+template <typename T>
+__device__ __forceinline__ void force_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                         T* __restrict__ w, const T* __restrict__ rho,
                                          const uint8_t* __restrict__ flags, const ForceArgs& a,
                                          int x, int y, int z)
 {
     if (x >= g.nx || y < 1 || y >= g.ny || z >= g.nz) return;
     int i = g.idx(x, y, z);
     uint8_t fl = flags[i];
-    float vv = v[i];
+    float vv = ldf(v, i);
     bool write_v = false;
     if ((fl & F_SELF) && (fl & F_YM)) {
-        float r = rho[i];
+        float r = ldf(rho, i);
         if (!a.gate_on || r > a.gate) {
             float b = a.buoyancy * (r - 0.0f);
             vv = vv + (r * a.gravity * a.dt + b * a.dt);
@@ -77,17 +98,17 @@ __device__ __forceinline__ void force_at(const Grid& g, float* __restrict__ u,
         }
     }
     if (x >= 1 && z >= 1) {
-        float uu = u[i];
-        float ww = w[i];
+        float uu = ldf(u, i);
+        float ww = ldf(w, i);
         float L = uu * uu + vv * vv + ww * ww;
         if (L * a.dt > a.max_v2) {
-            u[i] = uu * (a.max_v2 / (L * a.dt));
+            stf(u, i, uu * (a.max_v2 / (L * a.dt)));
             vv = vv * (a.max_v2 / (L * a.dt));
-            w[i] = ww * (a.max_v2 / (L * a.dt));
+            stf(w, i, ww * (a.max_v2 / (L * a.dt)));
             write_v = true;
         }
     }
-    if (write_v) v[i] = vv;
+    if (write_v) stf(v, i, vv);
 }
 
 // Colour-o cells of row (y,z): x = 2t + 1 + ((y+z+o+1) & 1) enumerates the
@@ -99,9 +120,9 @@ __device__ __forceinline__ int colour_x(int t, int y, int z, int o)
 
 // Right-hand side of the pressure form: d = div(u*) on active cells, in the
 // reference's operand order (smokeSimulation.cu:379-381); 0 elsewhere.
-__device__ __forceinline__ void divergence_at(const Grid& g, const float* __restrict__ u,
-                                              const float* __restrict__ v,
-                                              const float* __restrict__ w,
+template <typename T>
+__device__ __forceinline__ void divergence_at(const Grid& g, const T* __restrict__ u,
+                                              const T* __restrict__ v, const T* __restrict__ w,
                                               const uint8_t* __restrict__ flags,
                                               float* __restrict__ d, int x, int y, int z)
 {
@@ -109,7 +130,8 @@ __device__ __forceinline__ void divergence_at(const Grid& g, const float* __rest
     int i = g.idx(x, y, z);
     float dv = 0.0f;
     if (flags[i] & F_ACTIVE)
-        dv = -u[i] + u[i + 1] + -v[i] + v[i + g.px] + -w[i] + w[i + g.plane];
+        dv = -ldf(u, i) + ldf(u, i + 1) + -ldf(v, i) + ldf(v, i + g.px) + -ldf(w, i) +
+             ldf(w, i + g.plane);
     d[i] = dv;
 }
 
@@ -299,9 +321,9 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
 
 // u -= grad P on every face whose two cells are fluid.  Faces that touch no
 // active cell see P_b - P_a = 0 and are skipped (x,y,z >= 1 suffices).
-__device__ __forceinline__ void gradient_at(const Grid& g, float* __restrict__ u,
-                                            float* __restrict__ v, float* __restrict__ w,
-                                            const float* __restrict__ p,
+template <typename T>
+__device__ __forceinline__ void gradient_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                            T* __restrict__ w, const float* __restrict__ p,
                                             const uint8_t* __restrict__ flags, int x, int y, int z)
 {
     if (x < 1 || x > g.nx - 1 || y < 1 || y > g.ny - 1 || z < 1 || z > g.nz - 1) return;
@@ -309,29 +331,32 @@ __device__ __forceinline__ void gradient_at(const Grid& g, float* __restrict__ u
     uint8_t fl = flags[i];
     if (!(fl & F_SELF)) return;
     float pc = p[i];
-    if (fl & F_XM) u[i] = u[i] - (pc - p[i - 1]);
-    if (fl & F_YM) v[i] = v[i] - (pc - p[i - g.px]);
-    if (fl & F_ZM) w[i] = w[i] - (pc - p[i - g.plane]);
+    if (fl & F_XM) stf(u, i, ldf(u, i) - (pc - p[i - 1]));
+    if (fl & F_YM) stf(v, i, ldf(v, i) - (pc - p[i - g.px]));
+    if (fl & F_ZM) stf(w, i, ldf(w, i) - (pc - p[i - g.plane]));
 }
 
 // avgU/avgV/avgW (smokeSimulation.cu:409-447): 8-point means in the
 // reference's summation order.
-__device__ __forceinline__ float avgU(const float* __restrict__ a, int i, int px, int pl)
+template <typename T>
+__device__ __forceinline__ float avgU(const T* __restrict__ a, int i, int px, int pl)
 {
-    float t = a[i - pl] + a[i + 1 - pl] + a[i - px - pl] + a[i + 1 - px - pl] + a[i] + a[i + 1] +
-              a[i - px] + a[i + 1 - px];
+    float t = ldf(a, i - pl) + ldf(a, i + 1 - pl) + ldf(a, i - px - pl) + ldf(a, i + 1 - px - pl) +
+              ldf(a, i) + ldf(a, i + 1) + ldf(a, i - px) + ldf(a, i + 1 - px);
     return t / 8.0f;
 }
-__device__ __forceinline__ float avgV(const float* __restrict__ a, int i, int px, int pl)
+template <typename T>
+__device__ __forceinline__ float avgV(const T* __restrict__ a, int i, int px, int pl)
 {
-    float t = a[i - pl] + a[i - 1 - pl] + a[i + px - pl] + a[i - 1 + px - pl] + a[i] + a[i - 1] +
-              a[i + px] + a[i - 1 + px];
+    float t = ldf(a, i - pl) + ldf(a, i - 1 - pl) + ldf(a, i + px - pl) + ldf(a, i - 1 + px - pl) +
+              ldf(a, i) + ldf(a, i - 1) + ldf(a, i + px) + ldf(a, i - 1 + px);
     return t / 8.0f;
 }
-__device__ __forceinline__ float avgW(const float* __restrict__ a, int i, int px, int pl)
+template <typename T>
+__device__ __forceinline__ float avgW(const T* __restrict__ a, int i, int px, int pl)
 {
-    float t = a[i] + a[i - 1] + a[i - px] + a[i - 1 - px] + a[i - pl] + a[i - 1 - pl] +
-              a[i - px - pl] + a[i - 1 - px - pl];
+    float t = ldf(a, i) + ldf(a, i - 1) + ldf(a, i - px) + ldf(a, i - 1 - px) + ldf(a, i - pl) +
+              ldf(a, i - 1 - pl) + ldf(a, i - px - pl) + ldf(a, i - 1 - px - pl);
     return t / 8.0f;
 }
 
@@ -381,9 +406,14 @@ __device__ __forceinline__ float2 ld2(const float* __restrict__ a, int r)
     __builtin_memcpy(&v, a + r, sizeof v);
     return v;
 }
+__device__ __forceinline__ float2 ld2(const bf16s* __restrict__ a, int r)
+{
+    return make_float2(ldf(a, r), ldf(a, r + 1));
+}
 
 // The eight corners in the reference's order (:473-481).
-__device__ __forceinline__ float tri_eval(const float* __restrict__ a, const Tri& t)
+template <typename T>
+__device__ __forceinline__ float tri_eval(const T* __restrict__ a, const Tri& t)
 {
     const float2 p00 = ld2(a, t.r00), p10 = ld2(a, t.r10), p01 = ld2(a, t.r01),
                  p11 = ld2(a, t.r11);
@@ -406,11 +436,11 @@ __device__ __forceinline__ float tri_eval(const float* __restrict__ a, const Tri
 // all three samples.  Every address read lies inside the allocation (faces up
 // to x = nx, y = ny; clamped sample corners), so the loads are unconditional
 // and only the stores are predicated.
-__device__ __forceinline__ void advect_velocity_at(const Grid& g, const float* __restrict__ u0,
-                                                   const float* __restrict__ v0,
-                                                   const float* __restrict__ w0,
-                                                   float* __restrict__ u1, float* __restrict__ v1,
-                                                   float* __restrict__ w1,
+template <typename T>
+__device__ __forceinline__ void advect_velocity_at(const Grid& g, const T* __restrict__ u0,
+                                                   const T* __restrict__ v0,
+                                                   const T* __restrict__ w0, T* __restrict__ u1,
+                                                   T* __restrict__ v1, T* __restrict__ w1,
                                                    const uint8_t* __restrict__ flags, float dt,
                                                    int x, int y, int z)
 {
@@ -423,7 +453,7 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const float* _
     const bool doW = self && x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM);
     if (!(doU || doV || doW)) return;
     const int px = g.px, pl = g.plane;
-    const float uu = u0[i], vv = v0[i], ww = w0[i];
+    const float uu = ldf(u0, i), vv = ldf(v0, i), ww = ldf(w0, i);
     const float aU = avgU(u0, i, px, pl);
     const float aV = avgV(v0, i, px, pl);
     const float aW = avgW(w0, i, px, pl);
@@ -437,31 +467,30 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const float* _
     const float nu = tri_eval(u0, tu);
     const float nv = tri_eval(v0, tv);
     const float nw = tri_eval(w0, tw);
-    if (doU) u1[i] = nu;
-    if (doV) v1[i] = nv;
-    if (doW) w1[i] = nw;
+    if (doU) stf(u1, i, nu);
+    if (doV) stf(v1, i, nv);
+    if (doW) stf(w1, i, nw);
 }
 
 // advectSmoke (smokeSimulation.cu:617-638): cell velocity from the
 // post-advection faces, back-trace in double rounded to float.
-__device__ __forceinline__ void advect_density_at(const Grid& g, const float* __restrict__ r0,
-                                                  float* __restrict__ r1,
-                                                  const float* __restrict__ u,
-                                                  const float* __restrict__ v,
-                                                  const float* __restrict__ w,
+template <typename T>
+__device__ __forceinline__ void advect_density_at(const Grid& g, const T* __restrict__ r0,
+                                                  T* __restrict__ r1, const T* __restrict__ u,
+                                                  const T* __restrict__ v, const T* __restrict__ w,
                                                   const uint8_t* __restrict__ flags, float dt,
                                                   int x, int y, int z)
 {
     if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2) return;
     int i = g.idx(x, y, z);
     if (!(flags[i] & F_SELF)) return;
-    float ut = (u[i] + u[i + 1]) / 2.0f;
-    float vt = (v[i] + v[i + g.px]) / 2.0f;
-    float wt = (w[i] + w[i + g.plane]) / 2.0f;
+    float ut = (ldf(u, i) + ldf(u, i + 1)) / 2.0f;
+    float vt = (ldf(v, i) + ldf(v, i + g.px)) / 2.0f;
+    float wt = (ldf(w, i) + ldf(w, i + g.plane)) / 2.0f;
     float xt = (float)((double)x + 0.5 - (double)(ut * dt));
     float yt = (float)((double)y + 0.5 - (double)(vt * dt));
     float zt = (float)((double)z + 0.5 - (double)(wt * dt));
-    r1[i] = tri_eval(r0, tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f));
+    stf(r1, i, tri_eval(r0, tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f)));
 }
 
 }  // namespace dev

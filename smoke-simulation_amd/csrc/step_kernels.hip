@@ -1,4 +1,6 @@
-// HIP kernels of one smoke time step for one simulation, FP32, for gfx950.
+// HIP kernels of one smoke time step for one simulation, for gfx950: FP32
+// fields, or BF16 fields with FP32 arithmetic (the mixed-precision form; the
+// per-field kernels are templates over the storage element).
 //
 // The arithmetic lives in step_device.h (shared with the batched kernels);
 // the kernels here map threads to cells.  Thread blocks are 64 x 4 x 1: one
@@ -42,6 +44,23 @@ __device__ __forceinline__ long long cidx(int x, int y, int z, int rx, int ry)
     return x + (long long)rx * (y + (long long)ry * z);
 }
 
+// FP32 values of the reference layout <-> a padded field stored as T.
+template <bool TO_PADDED, typename T>
+__global__ void k_convert(Grid g, const void* __restrict__ src_, void* __restrict__ dst_, int wx,
+                          int wy, int wz)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= wx || y >= wy || z >= wz) return;
+    int i = g.idx(x, y, z);
+    long long c = cidx(x, y, z, wx, wy);
+    if (TO_PADDED)
+        dev::stf(static_cast<T*>(dst_), i, static_cast<const float*>(src_)[c]);
+    else
+        static_cast<float*>(dst_)[c] = dev::ldf(static_cast<const T*>(src_), i);
+}
+
 template <bool TO_PADDED, typename T>
 __global__ void k_repack(Grid g, const T* __restrict__ src, T* __restrict__ dst, int wx, int wy,
                          int wz)
@@ -59,22 +78,20 @@ __global__ void k_repack(Grid g, const T* __restrict__ src, T* __restrict__ dst,
 // Padded FP32 cells -> contiguous 16-bit values: format 1 FP16 (hardware
 // conversion, round to nearest even), format 2 BF16 (round to nearest even on
 // the FP32 bits; NaN stays a quiet NaN).
-__global__ void k_pack_half(Grid g, const float* __restrict__ src, uint16_t* __restrict__ dst,
-                            int format)
+template <typename T>
+__global__ void k_pack_half(Grid g, const T* __restrict__ src, uint16_t* __restrict__ dst, int format)
 {
     int x = blockIdx.x * BX + threadIdx.x;
     int y = blockIdx.y * BY + threadIdx.y;
     int z = blockIdx.z;
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
-    const float f = src[g.idx(x, y, z)];
+    const float f = dev::ldf(src, g.idx(x, y, z));
     uint16_t h;
     if (format == 1) {
         const _Float16 v = (_Float16)f;
         h = __builtin_bit_cast(uint16_t, v);
     } else {
-        const uint32_t b = __float_as_uint(f);
-        if ((b & 0x7fffffffu) > 0x7f800000u) h = (uint16_t)((b >> 16) | 0x0040u);
-        else h = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+        h = dev::bf16_rne(f);
     }
     dst[cidx(x, y, z, g.nx, g.ny)] = h;
 }
@@ -144,8 +161,9 @@ __global__ void k_flags(Grid g, const uint8_t* __restrict__ s, uint8_t* __restri
 
 // fillSmoke (smokeSimulation.cu:251-273) restricted to the bounding box of
 // one source (clipped to the interior by the launcher).
-__global__ void k_source(Grid g, float* __restrict__ r0, float* __restrict__ r1, float4 sph,
-                         int x0, int y0, int z0, int ex, int ey)
+template <typename T>
+__global__ void k_source(Grid g, T* __restrict__ r0, T* __restrict__ r1, float4 sph, int x0, int y0,
+                         int z0, int ex, int ey)
 {
     int x = x0 + blockIdx.x * BX + threadIdx.x;
     int y = y0 + blockIdx.y * BY + threadIdx.y;
@@ -156,8 +174,9 @@ __global__ void k_source(Grid g, float* __restrict__ r0, float* __restrict__ r1,
 
 // ---------------------------------------------------------- step stages
 
-__global__ void k_force(Grid g, float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
-                        const float* __restrict__ rho, const uint8_t* __restrict__ flags, ForceArgs a)
+template <typename T>
+__global__ void k_force(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                        const T* __restrict__ rho, const uint8_t* __restrict__ flags, ForceArgs a)
 {
     dev::force_at(g, u, v, w, rho, flags, a, blockIdx.x * BX + threadIdx.x,
                   blockIdx.y * BY + threadIdx.y + 1, blockIdx.z);
@@ -193,8 +212,9 @@ __global__ void k_project_ref_half(Grid g, float* __restrict__ u, float* __restr
     w[i + g.plane] += p * (float)sz1;
 }
 
-__global__ void k_divergence(Grid g, const float* __restrict__ u, const float* __restrict__ v,
-                             const float* __restrict__ w, const uint8_t* __restrict__ flags,
+template <typename T>
+__global__ void k_divergence(Grid g, const T* __restrict__ u, const T* __restrict__ v,
+                             const T* __restrict__ w, const uint8_t* __restrict__ flags,
                              float* __restrict__ d)
 {
     dev::divergence_at(g, u, v, w, flags, d, blockIdx.x * BX + threadIdx.x + 1,
@@ -258,27 +278,27 @@ __global__ void k_mask_pressure(Grid g, float* __restrict__ p, const uint8_t* __
     if (!(flags[i] & F_ACTIVE)) p[i] = 0.0f;
 }
 
-__global__ void k_gradient(Grid g, float* __restrict__ u, float* __restrict__ v,
-                           float* __restrict__ w, const float* __restrict__ p,
-                           const uint8_t* __restrict__ flags)
+template <typename T>
+__global__ void k_gradient(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                           const float* __restrict__ p, const uint8_t* __restrict__ flags)
 {
     dev::gradient_at(g, u, v, w, p, flags, blockIdx.x * BX + threadIdx.x + 1,
                      blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
 }
 
-__global__ void k_advect_velocity(Grid g, const float* __restrict__ u0, const float* __restrict__ v0,
-                                  const float* __restrict__ w0, float* __restrict__ u1,
-                                  float* __restrict__ v1, float* __restrict__ w1,
-                                  const uint8_t* __restrict__ flags, float dt)
+template <typename T>
+__global__ void k_advect_velocity(Grid g, const T* __restrict__ u0, const T* __restrict__ v0,
+                                  const T* __restrict__ w0, T* __restrict__ u1, T* __restrict__ v1,
+                                  T* __restrict__ w1, const uint8_t* __restrict__ flags, float dt)
 {
     dev::advect_velocity_at(g, u0, v0, w0, u1, v1, w1, flags, dt, blockIdx.x * BX + threadIdx.x + 1,
                             blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
 }
 
-__global__ void k_advect_density(Grid g, const float* __restrict__ r0, float* __restrict__ r1,
-                                 const float* __restrict__ u, const float* __restrict__ v,
-                                 const float* __restrict__ w, const uint8_t* __restrict__ flags,
-                                 float dt)
+template <typename T>
+__global__ void k_advect_density(Grid g, const T* __restrict__ r0, T* __restrict__ r1,
+                                 const T* __restrict__ u, const T* __restrict__ v,
+                                 const T* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
 {
     dev::advect_density_at(g, r0, r1, u, v, w, flags, dt, blockIdx.x * BX + threadIdx.x + 1,
                            blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
@@ -348,11 +368,49 @@ hipError_t pack_cells_u8(const Grid& g, const uint8_t* src, uint8_t* dst, hipStr
     return hipGetLastError();
 }
 
+template <typename T>
+static hipError_t launch_pack_half(const Grid& g, const T* src, uint16_t* dst, int format,
+                                   hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pack_half<T>, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, src, dst,
+                       format);
+    return hipGetLastError();
+}
 hipError_t pack_cells_half(const Grid& g, const float* src, uint16_t* dst, int format,
                            hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pack_half, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, src, dst,
-                       format);
+    return launch_pack_half(g, src, dst, format, st);
+}
+hipError_t pack_cells_half(const Grid& g, const bf16s* src, uint16_t* dst, int format,
+                           hipStream_t st)
+{
+    return launch_pack_half(g, src, dst, format, st);
+}
+
+hipError_t unpack_cells_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_convert<true, bf16s>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g,
+                       (const void*)src, (void*)dst, g.nx, g.ny, g.nz);
+    return hipGetLastError();
+}
+hipError_t pack_cells_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_convert<false, bf16s>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,
+                       g, (const void*)src, (void*)dst, g.nx, g.ny, g.nz);
+    return hipGetLastError();
+}
+hipError_t unpack_faces_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_convert<true, bf16s>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),
+                       dim3(BX, BY), 0, st, g, (const void*)src, (void*)dst, g.nx + 1, g.ny + 1,
+                       g.nz + 1);
+    return hipGetLastError();
+}
+hipError_t pack_faces_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_convert<false, bf16s>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),
+                       dim3(BX, BY), 0, st, g, (const void*)src, (void*)dst, g.nx + 1, g.ny + 1,
+                       g.nz + 1);
     return hipGetLastError();
 }
 
@@ -395,24 +453,47 @@ bool source_box(const Grid& g, float4 sph, int lo[3], int ext[3])
     return true;
 }
 
-hipError_t fill_source(const Grid& g, float* rho0, float* rho1, float4 sph, hipStream_t st,
-                       int* launched)
+template <typename T>
+static hipError_t launch_source(const Grid& g, T* rho0, T* rho1, float4 sph, hipStream_t st,
+                               int* launched)
 {
     *launched = 0;
     int lo[3], ext[3];
     if (!source_box(g, sph, lo, ext)) return hipSuccess;
-    hipLaunchKernelGGL(k_source, grid_for(ext[0], ext[1], ext[2]), dim3(BX, BY), 0, st, g, rho0,
+    hipLaunchKernelGGL(k_source<T>, grid_for(ext[0], ext[1], ext[2]), dim3(BX, BY), 0, st, g, rho0,
                        rho1, sph, lo[0], lo[1], lo[2], ext[0], ext[1]);
     *launched = 1;
     return hipGetLastError();
 }
 
+hipError_t fill_source(const Grid& g, float* rho0, float* rho1, float4 sph, hipStream_t st,
+                       int* launched)
+{
+    return launch_source(g, rho0, rho1, sph, st, launched);
+}
+hipError_t fill_source(const Grid& g, bf16s* rho0, bf16s* rho1, float4 sph, hipStream_t st,
+                       int* launched)
+{
+    return launch_source(g, rho0, rho1, sph, st, launched);
+}
+
+template <typename T>
+static hipError_t launch_force(const Grid& g, T* u, T* v, T* w, const T* rho, const uint8_t* flags,
+                               const ForceArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_force<T>, grid_for(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
+                       rho, flags, a);
+    return hipGetLastError();
+}
 hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
                  const uint8_t* flags, const ForceArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_force, grid_for(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
-                       rho, flags, a);
-    return hipGetLastError();
+    return launch_force(g, u, v, w, rho, flags, a, st);
+}
+hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const bf16s* rho,
+                 const uint8_t* flags, const ForceArgs& a, hipStream_t st)
+{
+    return launch_force(g, u, v, w, rho, flags, a, st);
 }
 
 hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
@@ -424,12 +505,23 @@ hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
     return hipGetLastError();
 }
 
+template <typename T>
+static hipError_t launch_divergence(const Grid& g, const T* u, const T* v, const T* w,
+                                    const uint8_t* flags, float* d, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_divergence<T>, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st,
+                       g, u, v, w, flags, d);
+    return hipGetLastError();
+}
 hipError_t divergence(const Grid& g, const float* u, const float* v, const float* w,
                       const uint8_t* flags, float* d, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_divergence, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
-                       u, v, w, flags, d);
-    return hipGetLastError();
+    return launch_divergence(g, u, v, w, flags, d, st);
+}
+hipError_t divergence(const Grid& g, const bf16s* u, const bf16s* v, const bf16s* w,
+                      const uint8_t* flags, float* d, hipStream_t st)
+{
+    return launch_divergence(g, u, v, w, flags, d, st);
 }
 
 hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const float* d,
@@ -506,30 +598,66 @@ hipError_t mask_pressure(const Grid& g, float* p, const uint8_t* flags, hipStrea
     return hipGetLastError();
 }
 
+template <typename T>
+static hipError_t launch_gradient(const Grid& g, T* u, T* v, T* w, const float* p,
+                                  const uint8_t* flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gradient<T>, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st,
+                       g, u, v, w, p, flags);
+    return hipGetLastError();
+}
 hipError_t gradient(const Grid& g, float* u, float* v, float* w, const float* p,
                     const uint8_t* flags, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gradient, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st, g,
-                       u, v, w, p, flags);
-    return hipGetLastError();
+    return launch_gradient(g, u, v, w, p, flags, st);
+}
+hipError_t gradient(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* p,
+                    const uint8_t* flags, hipStream_t st)
+{
+    return launch_gradient(g, u, v, w, p, flags, st);
 }
 
+template <typename T>
+static hipError_t launch_advect_velocity(const Grid& g, const T* u0, const T* v0, const T* w0,
+                                         T* u1, T* v1, T* w1, const uint8_t* flags, float dt,
+                                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advect_velocity<T>, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY),
+                       0, st, g, u0, v0, w0, u1, v1, w1, flags, dt);
+    return hipGetLastError();
+}
 hipError_t advect_velocity(const Grid& g, const float* u0, const float* v0, const float* w0,
                            float* u1, float* v1, float* w1, const uint8_t* flags, float dt,
                            hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advect_velocity, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0,
-                       st, g, u0, v0, w0, u1, v1, w1, flags, dt);
-    return hipGetLastError();
+    return launch_advect_velocity(g, u0, v0, w0, u1, v1, w1, flags, dt, st);
+}
+hipError_t advect_velocity(const Grid& g, const bf16s* u0, const bf16s* v0, const bf16s* w0,
+                           bf16s* u1, bf16s* v1, bf16s* w1, const uint8_t* flags, float dt,
+                           hipStream_t st)
+{
+    return launch_advect_velocity(g, u0, v0, w0, u1, v1, w1, flags, dt, st);
 }
 
+template <typename T>
+static hipError_t launch_advect_density(const Grid& g, const T* r0, T* r1, const T* u, const T* v,
+                                        const T* w, const uint8_t* flags, float dt, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advect_density<T>, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
+                       st, g, r0, r1, u, v, w, flags, dt);
+    return hipGetLastError();
+}
 hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float* u,
                           const float* v, const float* w, const uint8_t* flags, float dt,
                           hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advect_density, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
-                       st, g, r0, r1, u, v, w, flags, dt);
-    return hipGetLastError();
+    return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
+}
+hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s* u,
+                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
+                          hipStream_t st)
+{
+    return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
 }
 
 }  // namespace smk

@@ -20,7 +20,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsmokesim.so")
 
 SMOKE_OK = 0
-PROJ_REFERENCE, PROJ_RBSOR, PROJ_JACOBI = 0, 1, 2
+PROJ_REFERENCE, PROJ_RBSOR, PROJ_JACOBI, PROJ_REFINED = 0, 1, 2, 3
+STORAGE_FP32, STORAGE_BF16 = 0, 1
 OBSTACLES_LAST_WINS, OBSTACLES_UNION = 0, 1
 STAGE_SCENE, STAGE_FORCE, STAGE_PROJECT, STAGE_ADVECT_VEL, STAGE_ADVECT_DENS = range(5)
 STAGE_COUNT = 5
@@ -49,6 +50,9 @@ class Params(ctypes.Structure):
         ("iterations", ctypes.c_int),
         ("obstacle_mode", ctypes.c_int),
         ("warm_start", ctypes.c_int),
+        ("inner_sweeps", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("tolerance", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -84,9 +88,13 @@ def _declare(lib):
         "smoke_device_info": (st, [ctypes.c_char_p, ctypes.c_size_t]),
         "smoke_params_default": (st, [pp]),
         "smoke_create": (st, [u, u, u, fp, ctypes.POINTER(vp)]),
+        "smoke_create_ex": (st, [u, u, u, fp, i, ctypes.POINTER(vp)]),
+        "smoke_get_storage": (i, [vp]),
         "smoke_destroy": (None, [vp]),
         "smoke_get_dims": (st, [vp, up, up, up]),
         "smoke_footprint_bytes": (ctypes.c_size_t, [u, u, u]),
+        "smoke_footprint_bytes_ex": (ctypes.c_size_t, [u, u, u, i, i]),
+        "smoke_get_solve_info": (st, [vp, dp, ip]),
         "smoke_set_params": (st, [vp, pp]),
         "smoke_get_params": (st, [vp, pp]),
         "smoke_set_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
@@ -187,14 +195,15 @@ def default_params():
     return p.as_dict()
 
 
-def footprint_bytes(nx, ny, nz):
-    return int(load().smoke_footprint_bytes(nx, ny, nz))
+def footprint_bytes(nx, ny, nz, storage=STORAGE_FP32, refined=False):
+    return int(load().smoke_footprint_bytes_ex(nx, ny, nz, int(storage), 1 if refined else 0))
 
 
 class Simulation:
-    """One simulation on the current device (smoke_create ... smoke_destroy)."""
+    """One simulation on the current device (smoke_create_ex ... smoke_destroy).
+    storage: STORAGE_FP32 (the drop-in) or STORAGE_BF16 (mixed-precision form)."""
 
-    def __init__(self, nx, ny, nz, density=None, params=None):
+    def __init__(self, nx, ny, nz, density=None, params=None, storage=STORAGE_FP32):
         self.lib = load()
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         h = ctypes.c_void_p()
@@ -202,10 +211,18 @@ class Simulation:
         if density is not None:
             d = np.ascontiguousarray(density, dtype=np.float32).reshape(self.cell_shape)
             dptr = _fptr(d)
-        _check(self.lib.smoke_create(self.nx, self.ny, self.nz, dptr, ctypes.byref(h)))
+        _check(self.lib.smoke_create_ex(self.nx, self.ny, self.nz, dptr, int(storage),
+                                        ctypes.byref(h)))
         self.h = h
+        self.storage = int(storage)
         if params:
             self.set_params(**params)
+
+    def solve_info(self):
+        """(relative residual, refinements) of the last REFINED solve."""
+        rel, n = ctypes.c_double(), ctypes.c_int()
+        _check(self.lib.smoke_get_solve_info(self.h, ctypes.byref(rel), ctypes.byref(n)))
+        return rel.value, n.value
 
     def close(self):
         h = getattr(self, "h", None)

@@ -149,8 +149,57 @@ def case_readback():
     g.close()
 
 
+def _rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a))
+
+
+def case_precision():
+    # REFINED (FP32 and BF16 fields) against FP64 Jacobi with the same number
+    # of sweeps, BF16 conversions, and the tolerance-driven loop.
+    sc = scenes.test_scene((18, 14, 12), seed=7, projection=2, iterations=12)
+    o = oracle_for(sc, 64)
+    o.step(sc["dt"], 2)
+    ref = o.state()
+    for storage in (smokesim.STORAGE_FP32, smokesim.STORAGE_BF16):
+        g = gpu_for(sc, storage=storage)
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=6, iterations=12)
+        for _ in range(2):
+            g.step(sc["dt"], readback=False)
+        st = g.state()
+        rel, nref = g.solve_info()
+        assert nref == 2 and 0.0 < rel < 1.0, (rel, nref)
+        assert np.array_equal(st["flags"], ref["flags"])
+        for k in ("rho0", "rho1", "u0", "u1", "v0", "v1", "w0", "w1"):
+            e = _rel_l2(st[k], ref[k])
+            assert e < 3e-2, (storage, k, e)
+        e = _rel_l2(g.get_pressure(), o.get_pressure())
+        assert e < 3e-2, (storage, "pressure", e)
+        g.close()
+    # BF16 round trip: stored values are the round-to-nearest-even BF16 of the input
+    g = smokesim.Simulation(9, 7, 5, storage=smokesim.STORAGE_BF16)
+    x = np.random.default_rng(1).standard_normal((5, 7, 9)).astype(np.float32)
+    g.set_density(0, x)
+    b = x.view(np.uint32).astype(np.uint64)
+    rne = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16).astype(np.uint32).view(np.float32)
+    same_bits(g.get_density(0), rne, "bf16 store rounding")
+    g.close()
+    # tolerance-driven refinement on a small grid
+    sc = scenes.test_scene((8, 7, 6), seed=8, projection=2, iterations=10, box_fraction=0.0,
+                           n_obstacles=0)
+    g = gpu_for(sc)
+    g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=10, iterations=2000,
+                 tolerance=1e-5)
+    g.step(sc["dt"], readback=False)
+    rel, nref = g.solve_info()
+    assert rel <= 1e-5 and 1 <= nref < 1000, (rel, nref)
+    g.close()
+
+
 CASES = {"single": case_single, "plans": case_plans, "batch": case_batch,
-         "server": case_server, "readback": case_readback}
+         "server": case_server, "readback": case_readback, "precision": case_precision}
 
 
 def main():

@@ -70,9 +70,13 @@ void check(hipEvent_t e)
     if (!e || e->magic != kEvent) die("use of an invalid event");
 }
 
+unsigned long long g_exchange[1024];
+
 void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 block)
 {
-    const bool threaded = strstr(name, "jacobi_fused") || strstr(name, "kb_jacobi<");
+    // Kernels with barriers or lane exchanges run their threads concurrently.
+    const bool threaded =
+        strstr(name, "jacobi_fused") || strstr(name, "kb_jacobi<") || strstr(name, "k_ref_");
     const unsigned nt = block.x * block.y * block.z;
     const EmuIdx gd{grid.x, grid.y, grid.z}, bd{block.x, block.y, block.z};
     auto set_thread = [&](unsigned t) {
@@ -121,6 +125,17 @@ void emu_syncthreads()
 {
     if (!g_threaded) die("__syncthreads in a kernel run without threads");
     g_barrier->arrive_and_wait();
+}
+
+unsigned long long emu_exchange(unsigned long long bits, int partner)
+{
+    if (!g_threaded) die("lane exchange in a kernel run without threads");
+    const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    g_exchange[t] = bits;
+    g_barrier->arrive_and_wait();
+    const unsigned long long r = g_exchange[partner];
+    g_barrier->arrive_and_wait();
+    return r;
 }
 
 hipError_t emu_last_error() { return hipSuccess; }

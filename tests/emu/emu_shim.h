@@ -32,6 +32,30 @@ inline unsigned __float_as_uint(float f)
     std::memcpy(&u, &f, 4);
     return u;
 }
+inline float __uint_as_float(unsigned u)
+{
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Lane exchange of a block run on host threads: every thread of the block
+// publishes its value and reads its partner's (both sides of a barrier).
+unsigned long long emu_exchange(unsigned long long bits, int partner);
+template <typename T>
+T __shfl_xor(T v, int mask, int width = 64)
+{
+    static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+    const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    const int lane = t % width;
+    const int partner = (lane ^ mask) < width ? t - lane + (lane ^ mask) : t;
+    unsigned long long b = 0;
+    std::memcpy(&b, &v, sizeof(T));
+    b = emu_exchange(b, partner);
+    T r;
+    std::memcpy(&r, &b, sizeof(T));
+    return r;
+}
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 

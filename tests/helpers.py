@@ -15,10 +15,10 @@ def oracle_for(scene, real_bits, density=None):
     return o
 
 
-def gpu_for(scene, density=None):
+def gpu_for(scene, density=None, storage=0):
     import smokesim
     nx, ny, nz = scene["dims"]
-    s = smokesim.Simulation(nx, ny, nz)
+    s = smokesim.Simulation(nx, ny, nz, storage=storage)
     scenes.apply(s, scene, density)
     return s
 

@@ -1,0 +1,145 @@
+"""Mixed-precision form: BF16 storage of density and u, v, w (FP32
+arithmetic) and the REFINED projection (FP32 residual-correction loop around
+FP16 Jacobi sweeps, FP64 norms on the GPU), against the FP64 oracle.
+
+In exact arithmetic R refinements of n Jacobi sweeps from a zero correction
+are R*n Jacobi sweeps, so the FP64 oracle's Jacobi projection with R*n sweeps
+is the reference; the differences below are rounding only.  Tolerances
+(relative L2 over the whole field, FP64):
+  * FP32 fields + REFINED, 5 steps:  pressure and velocity <= 1e-3, density <= 1e-4
+  * BF16 fields (JACOBI or REFINED), 5 steps: density <= 5e-3, velocity and
+    pressure <= 2e-2 (BF16 keeps 8 significant bits: one rounding is <= 2^-9)
+  * the solid mask and the cell flags stay bit-exact in every form."""
+import numpy as np
+import pytest
+
+import smokesim
+from helpers import gpu_for, oracle_for, rel_l2
+from smokesim import scenes
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((33, 17, 21), 2), ((24, 20, 28), 3), ((40, 36, 30), 4)]
+
+
+def run_pair(dims, seed, storage, projection, inner=10, refinements=3, steps=5):
+    sc = scenes.test_scene(dims, seed=seed, projection=smokesim.PROJ_JACOBI,
+                           iterations=inner * refinements)
+    o = oracle_for(sc, 64)
+    o.step(sc["dt"], steps)
+    g = gpu_for(sc, storage=storage)
+    if projection == smokesim.PROJ_REFINED:
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=inner,
+                     iterations=inner * refinements)
+    for _ in range(steps):
+        g.step(sc["dt"], readback=False)
+    return g, o
+
+
+def field_errors(g, o):
+    a, b = g.state(), o.state()
+    assert np.array_equal(a["mask"], b["mask"]) and np.array_equal(a["flags"], b["flags"])
+    err = {"pressure": rel_l2(g.get_pressure(), o.get_pressure())}
+    for k in ("rho", "u", "v", "w"):
+        h = 1 - a["now"]  # the half the last step wrote
+        err[k] = rel_l2(a[f"{k}{h}"], b[f"{k}{h}"])
+    return err
+
+
+@pytest.mark.parametrize("dims,seed", CASES)
+def test_refined_fp32_fields_match_fp64_jacobi(dims, seed):
+    g, o = run_pair(dims, seed, smokesim.STORAGE_FP32, smokesim.PROJ_REFINED)
+    err = field_errors(g, o)
+    rel, nref = g.solve_info()
+    assert nref == 3 and 0.0 < rel < 1.0
+    assert err["pressure"] <= 1e-3 and max(err["u"], err["v"], err["w"]) <= 1e-3, err
+    assert err["rho"] <= 1e-4, err
+    g.close()
+
+
+@pytest.mark.parametrize("projection", [smokesim.PROJ_JACOBI, smokesim.PROJ_REFINED])
+@pytest.mark.parametrize("dims,seed", CASES[:2])
+def test_bf16_fields_within_tolerance(dims, seed, projection):
+    g, o = run_pair(dims, seed, smokesim.STORAGE_BF16, projection)
+    err = field_errors(g, o)
+    assert err["rho"] <= 5e-3, err
+    assert max(err["u"], err["v"], err["w"], err["pressure"]) <= 2e-2, err
+    g.close()
+
+
+def test_bf16_store_is_round_to_nearest_even():
+    g = smokesim.Simulation(19, 7, 5, storage=smokesim.STORAGE_BF16)
+    x = np.random.default_rng(3).standard_normal((5, 7, 19)).astype(np.float32)
+    x.flat[:4] = [1.0, -0.0, np.float32(1.00390625), np.float32(1.01171875)]  # exact, -0, ties
+    g.set_density(0, x)
+    b = x.view(np.uint32).astype(np.uint64)
+    rne = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16).astype(np.uint32)
+    assert np.array_equal(g.get_density(0).view(np.uint32), rne)
+    u = np.random.default_rng(4).standard_normal((6, 8, 20)).astype(np.float32)
+    g.set_velocity(1, 1, u)
+    bu = u.view(np.uint32).astype(np.uint64)
+    assert np.array_equal(g.get_velocity(1, 1).view(np.uint32),
+                          ((bu + 0x7FFF + ((bu >> 16) & 1)) >> 16 << 16).astype(np.uint32))
+    g.close()
+
+
+def test_refinement_reaches_the_tolerance():
+    """At 32^3 the loop runs until ||r||/||div|| <= 1e-5 (about 2.3 n^2
+    sweeps for Jacobi); the result matches FP64 Jacobi run for the same number
+    of sweeps, and a looser tolerance stops earlier."""
+    sc = scenes.test_scene((32, 32, 32), seed=11, projection=smokesim.PROJ_JACOBI, iterations=10,
+                           box_fraction=0.0, n_obstacles=0)
+    counts = []
+    for tol in (1e-3, 1e-5):
+        g = gpu_for(sc)
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=20, iterations=40000,
+                     tolerance=tol)
+        g.step(sc["dt"], readback=False)
+        rel, nref = g.solve_info()
+        assert rel <= tol and nref < 2000, (tol, rel, nref)
+        counts.append(nref)
+        if tol == 1e-5:
+            o = oracle_for(dict(sc, params=dict(sc["params"], iterations=20 * nref)), 64)
+            o.step(sc["dt"], 1)
+            assert rel_l2(g.get_pressure(), o.get_pressure()) <= 1e-3
+        g.close()
+    assert counts[0] < counts[1]
+
+
+def test_refined_and_bf16_argument_checks():
+    g = smokesim.Simulation(12, 10, 8, storage=smokesim.STORAGE_BF16)
+    assert g.get_params()["projection"] == smokesim.PROJ_JACOBI
+    with pytest.raises(smokesim.SmokeError):
+        g.set_params(projection=smokesim.PROJ_REFERENCE)
+    with pytest.raises(smokesim.SmokeError):
+        g.solve_info()  # the refined solver never ran
+    with pytest.raises(smokesim.SmokeError):
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=7)
+    with pytest.raises(smokesim.SmokeError):
+        g.set_params(projection=smokesim.PROJ_REFINED, warm_start=1)
+    with pytest.raises(smokesim.SmokeError):
+        g.set_params(projection=smokesim.PROJ_REFINED, tolerance=-1.0)
+    with pytest.raises(smokesim.SmokeError):
+        smokesim.Batch([g], [0.05])
+    g.close()
+    assert smokesim.footprint_bytes(64, 64, 64, smokesim.STORAGE_BF16) < smokesim.footprint_bytes(64, 64, 64)
+    assert smokesim.footprint_bytes(64, 64, 64, refined=True) > smokesim.footprint_bytes(64, 64, 64)
+
+
+def test_mixed_form_at_256_stays_close_to_fp32():
+    """Full size: BF16 + REFINED (5 x 20 sweeps) against the FP32 Jacobi-100
+    drop-in on the GPU after 5 steps (same inputs)."""
+    sc = scenes.grid_256()
+    a = gpu_for(sc)
+    b = gpu_for(sc, storage=smokesim.STORAGE_BF16)
+    b.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=20, iterations=100)
+    for _ in range(5):
+        a.step(sc["dt"], readback=False)
+        b.step(sc["dt"], readback=False)
+    da, db = a.read_density(), b.read_density()
+    assert np.isfinite(db).all()
+    assert rel_l2(db, da) <= 1e-2
+    rel, nref = b.solve_info()
+    assert nref == 5 and rel < 1.0
+    a.close()
+    b.close()
