@@ -53,15 +53,33 @@ def describe(sc):
     p = sc["params"]
     nx, ny, nz = sc["dims"]
     proj = {0: "red-black SOR (reference, on faces)", 1: "red-black SOR (pressure)",
-            2: "Jacobi"}[p["projection"]]
+            2: "Jacobi", 3: f"refined (FP32 outer, FP16 inner x{p.get('inner_sweeps')} per "
+                            "refinement, FP64 norms), total sweeps"}[p["projection"]]
     solid = "floor + seeded boxes 5%" if sc.get("solid") is not None else "floor"
-    return (f"{sc['name']}: {nx}x{ny}x{nz} FP32, {solid}, {len(sc['obstacles'])} obstacle(s), "
+    fields = "BF16 fields, FP32 math" if p["projection"] == 3 else "FP32"
+    return (f"{sc['name']}: {nx}x{ny}x{nz} {fields}, {solid}, {len(sc['obstacles'])} obstacle(s), "
             f"{len(sc['sources'])} source(s), {proj} x{p['iterations']}, dt={sc['dt']}")
 
 
-def step_bytes_per_cell(params):
-    """Algorithmic bytes per cell of one full step for the chosen projection."""
+BYTES_REF_SWEEP = 8            # FP16 sweep: e, qi, inv in; e out (2 B each)
+BYTES_REF_RESID = 19           # x, d (4), e (2), flags in; x, r out (4 each)
+BYTES_REF_NORM0 = 5            # d, flags in
+# BF16 fields: force 15, divergence 11, gradient 17, velocity advection 13,
+# density advection 11.
+BYTES_FIXED_BF16 = 15 + 11 + 17 + 13 + 11
+
+
+def step_bytes_per_cell(params, storage=0):
+    """Algorithmic bytes per cell of one full step for the chosen projection
+    and field storage (0 FP32, 1 BF16)."""
     k = params["iterations"]
+    if params["projection"] == 3:
+        inner = params.get("inner_sweeps", 20)
+        refinements = -(-k // inner)
+        fixed = BYTES_FIXED_BF16 if storage == 1 else BYTES_FIXED
+        return fixed + BYTES_REF_NORM0 + refinements * (inner * BYTES_REF_SWEEP + BYTES_REF_RESID)
+    if storage == 1:
+        return BYTES_FIXED_BF16 + BYTES_SWEEP * (k if params["projection"] == 2 else 2 * k)
     if params["projection"] == 0:
         # In-place red-black half-sweep on faces: every u, v, w face is read
         # and written once per half-sweep, plus the flags byte.
@@ -102,6 +120,61 @@ def latency(ms):
             "p99": round(float(np.percentile(ms, 99)), 4)}
 
 
+MIXED = dict(projection=3, inner_sweeps=20)   # with BF16 fields
+
+
+def error_check(sc, steps, mixed_iterations):
+    """Per-field relative L2 and max-abs error after `steps` steps of the FP32
+    form (Jacobi) and of the mixed form (BF16 fields, REFINED) against the
+    FP64 oracle (Jacobi, same sweep count), all from the same inputs."""
+    import numpy as np
+    import smokesim
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from smokesim import scenes
+    nx, ny, nz = sc["dims"]
+    forms = {"fp32": (smokesim.STORAGE_FP32, {}),
+             "mixed": (smokesim.STORAGE_BF16, dict(MIXED, iterations=mixed_iterations))}
+    got = {}
+    for name, (storage, extra) in forms.items():
+        s = smokesim.Simulation(nx, ny, nz, storage=storage)
+        scenes.apply(s, sc)
+        if extra:
+            s.set_params(**extra)
+        for _ in range(steps):
+            s.step_async(sc["dt"])
+        s.synchronize()
+        h = 1 - s.index_now
+        got[name] = {"rho": s.get_density(h), "u": s.get_velocity(0, h), "v": s.get_velocity(1, h),
+                     "w": s.get_velocity(2, h)}
+        if extra:
+            got[name]["relative_residual"] = s.solve_info()[0]
+        s.close()
+        log(f"error check: {name} form stepped {steps} steps")
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    o = orc.Oracle(nx, ny, nz, 64)
+    scenes.apply(o, sc)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        o.step(sc["dt"], 1)
+        log(f"error check: FP64 oracle step {k + 1}/{steps} ({time.perf_counter() - t0:.0f} s)")
+    h = 1 - o.index_now
+    ref = {"rho": o.get_density(h), "u": o.get_velocity(0, h), "v": o.get_velocity(1, h),
+           "w": o.get_velocity(2, h)}
+    out = {"steps": steps, "reference": "FP64 oracle, Jacobi x%d" % sc["params"]["iterations"]}
+    for name in forms:
+        errs = {}
+        for f, r in ref.items():
+            a = got[name][f].astype(np.float64)
+            d = np.linalg.norm(r.ravel())
+            errs[f] = {"rel_l2": float(np.linalg.norm((a - r).ravel()) / (d if d > 0 else 1.0)),
+                       "max_abs": float(np.abs(a - r).max())}
+        if "relative_residual" in got[name]:
+            errs["relative_residual"] = got[name]["relative_residual"]
+        out[name] = errs
+    return out
+
+
 def run_single(args, sc):
     import numpy as np
     import smokesim
@@ -109,11 +182,16 @@ def run_single(args, sc):
         sc["params"]["projection"] = {"reference": 0, "rbsor": 1, "jacobi": 2}[args.projection]
     if args.iterations is not None:
         sc["params"]["iterations"] = args.iterations
+    mixed = args.precision == "mixed"
+    storage = smokesim.STORAGE_BF16 if mixed else smokesim.STORAGE_FP32
     nx, ny, nz = sc["dims"]
     cells = nx * ny * nz
+    fp32_iterations = sc["params"]["iterations"]
+    if mixed:
+        sc["params"].update(MIXED)
     log("workload:", describe(sc))
 
-    sim = smokesim.Simulation(nx, ny, nz)
+    sim = smokesim.Simulation(nx, ny, nz, storage=storage)
     smokesim.scenes.apply(sim, sc)
     dt = sc["dt"]
     tuned = None
@@ -145,7 +223,8 @@ def run_single(args, sc):
 
     ms_per_step = 1000.0 * wall / args.steps
     p = sc["params"]
-    bpc = step_bytes_per_cell(p)
+    bpc = step_bytes_per_cell(p, storage)
+    solve = sim.solve_info() if p["projection"] == 3 else None
 
     # Dominant kernel: the pressure sweep.  Algorithmic bytes per launch =
     # 13 B x interior cells (the sweep's domain) x sweeps per launch;
@@ -155,7 +234,15 @@ def run_single(args, sc):
     roofline = None
     if sweep_launches > 0:
         avg_ms = sweep_ms / sweep_launches
-        if p["projection"] == 2:
+        if p["projection"] == 3:
+            # The sweep loop also holds the residual passes; their share is
+            # counted with their own bytes (19 B/cell) so the figure stays the
+            # loop's algorithmic rate.
+            refinements = -(-p["iterations"] // p["inner_sweeps"])
+            per_step = refinements * (p["inner_sweeps"] * BYTES_REF_SWEEP + BYTES_REF_RESID)
+            alg = per_step * interior * timing["steps"] / sweep_launches
+            kernel = "k_ref_sweeps"
+        elif p["projection"] == 2:
             per_launch = p["iterations"] * timing["steps"] / sweep_launches
             alg = BYTES_SWEEP * interior * per_launch
             kernel = "k_jacobi_fused"
@@ -194,12 +281,14 @@ def run_single(args, sc):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16 fields, f16 inner sweeps, f32 math" if mixed else "f32",
         "data": "synthetic",
         "config": {
             "workload": describe(sc), "cells": cells, "projection": p["projection"],
             "iterations": p["iterations"], "parallelism": "single-gpu",
+            "precision": args.precision,
         },
+        "solve": ({"relative_residual": solve[0], "refinements": solve[1]} if solve else None),
         "latency_ms": latency(step_ms),
         "stage_ms_per_step": {k: round(v / max(timing["steps"], 1), 4)
                               for k, v in timing["stage_ms"].items()},
@@ -209,10 +298,14 @@ def run_single(args, sc):
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    sim.close()
+    if args.error_steps:
+        esc = dict(sc, params=dict(sc["params"], projection=2, iterations=fp32_iterations))
+        out["accuracy"] = error_check(esc, args.error_steps, p["iterations"] if mixed else fp32_iterations)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 50)
-    sim.close()
+        bsc = dict(sc, params=dict(sc["params"], projection=2, iterations=fp32_iterations)) if mixed else sc
+        out["cpu_baseline"] = cpu_baseline(bsc, args.cpu_budget, 50)
     return out
 
 
@@ -345,6 +438,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default Jacobi plan")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32",
+                    help="mixed: BF16 fields, REFINED projection (FP16 inner sweeps, 20 per "
+                         "refinement, same total sweep count)")
+    ap.add_argument("--error-steps", type=int, default=0,
+                    help="also step the FP32 and mixed forms and the FP64 oracle this many steps "
+                         "from the same inputs and report per-field errors")
     ap.add_argument("--requests", type=int, default=200, help="serve: number of requests")
     ap.add_argument("--capacity", type=float, default=5.0e9,
                     help="serve: cell-updates/s the arrival rate is 80%% of")

@@ -83,27 +83,40 @@ __device__ __forceinline__ void block_partials(double s2, float m, double* __res
     }
 }
 
+// Planes per thread of the norm and residual passes (one block: 64 x 4 cells
+// x kZB planes, one partial): enough independent loads per thread to cover
+// the memory latency, and enough blocks to fill the GPU.
+constexpr int kZB = 4;
+
 // Norms of the right-hand side (the first residual, x = 0): r = d on active
-// cells.  Blocks cover (x, y) tiles and a run of `zper` planes.
+// cells.
 __global__ __launch_bounds__(BX * BY) void k_ref_norm0(Grid g, const float* __restrict__ d,
                                                        const uint8_t* __restrict__ flags,
                                                        double* __restrict__ part,
-                                                       unsigned* __restrict__ pmax, int zper)
+                                                       unsigned* __restrict__ pmax)
 {
     const int x = blockIdx.x * BX + threadIdx.x + 1;
     const int y = blockIdx.y * BY + threadIdx.y + 1;
-    const int zb = blockIdx.z * zper + 1, ze = min(zb + zper, g.nz - 1);
+    const int z0 = blockIdx.z * kZB + 1;
     double s2 = 0.0;
     float m = 0.0f;
-    if (x <= g.nx - 2 && y <= g.ny - 2)
-        for (int z = zb; z < ze; ++z) {
-            const int i = g.idx(x, y, z);
-            if (flags[i] & F_ACTIVE) {
-                const float r = d[i];
-                s2 += (double)r * (double)r;
-                m = fmaxf(m, fabsf(r));
+    if (x <= g.nx - 2 && y <= g.ny - 2) {
+        float r[kZB];
+#pragma unroll
+        for (int k = 0; k < kZB; ++k) {
+            const int z = z0 + k;
+            r[k] = 0.0f;
+            if (z <= g.nz - 2) {
+                const int i = g.idx(x, y, z);
+                r[k] = (flags[i] & F_ACTIVE) ? d[i] : 0.0f;
             }
         }
+#pragma unroll
+        for (int k = 0; k < kZB; ++k) {
+            s2 += (double)r[k] * (double)r[k];
+            m = fmaxf(m, fabsf(r[k]));
+        }
+    }
     block_partials(s2, m, part, pmax);
 }
 
@@ -111,7 +124,8 @@ __global__ __launch_bounds__(BX * BY) void k_ref_norm0(Grid g, const float* __re
 // next negated residual r = (d + acc*x_new) - sum_n x_new_n on active cells,
 // with its partial norms; LAST skips the r store.  Neighbour values are
 // recomputed with the same two operations, so they equal the neighbours' own
-// x_new.  Once `done` is set the pass only copies x_old.
+// x_new.  A thread walks kZB planes of one column keeping the column's
+// values in registers.  Once `done` is set the pass only copies x_old.
 template <bool XZERO, bool LAST>
 __global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __restrict__ xo,
                                                        float* __restrict__ xn,
@@ -121,43 +135,89 @@ __global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __re
                                                        float* __restrict__ q,
                                                        const RefineCtl* __restrict__ ctl,
                                                        double* __restrict__ part,
-                                                       unsigned* __restrict__ pmax, int zper)
+                                                       unsigned* __restrict__ pmax)
 {
     const int done = ctl->done;
     const float sinv = ctl->sinv;
     const int x = blockIdx.x * BX + threadIdx.x + 1;
     const int y = blockIdx.y * BY + threadIdx.y + 1;
-    const int zb = blockIdx.z * zper + 1, ze = min(zb + zper, g.nz - 1);
+    const int z0 = blockIdx.z * kZB + 1;
+    const int nk = min(kZB, g.nz - 1 - z0);  // planes of this block (>= 1)
     double s2 = 0.0;
     float m = 0.0f;
     if (x <= g.nx - 2 && y <= g.ny - 2) {
-        for (int z = zb; z < ze; ++z) {
-            const int i = g.idx(x, y, z);
-            if (done) {
-                xn[i] = XZERO ? 0.0f : xo[i];
-                continue;
-            }
+        const int i0 = g.idx(x, y, z0);
+        if (done) {
+#pragma unroll
+            for (int k = 0; k < kZB; ++k)
+                if (k < nk) xn[i0 + k * g.plane] = XZERO ? 0.0f : xo[i0 + k * g.plane];
+        } else {
             auto X = [&](int j) { return (XZERO ? 0.0f : xo[j]) + h2f(e[j]) * sinv; };
-            const float xc = X(i);
-            xn[i] = xc;
-            const uint8_t fl = flags[i];
-            float r = 0.0f;
-            if (fl & F_ACTIVE) {
-                const float sum = dev::sum6(X(i - 1), X(i + 1), X(i - g.px), X(i + g.px),
-                                            X(i - g.plane), X(i + g.plane));
-                r = (d[i] + (float)__popc(fl & F_NBRS) * xc) - sum;
-                s2 += (double)r * (double)r;
-                m = fmaxf(m, fabsf(r));
+            // Column values at planes z0-1 .. z0+kZB (clamped reads stay inside
+            // the padded field: planes 0..nz exist).
+            float col[kZB + 2];
+#pragma unroll
+            for (int k = 0; k < kZB + 2; ++k) col[k] = (k <= nk + 1) ? X(i0 + (k - 1) * g.plane) : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kZB; ++k) {
+                if (k >= nk) break;
+                const int i = i0 + k * g.plane;
+                const float xc = col[k + 1];
+                xn[i] = xc;
+                const uint8_t fl = flags[i];
+                float r = 0.0f;
+                if (fl & F_ACTIVE) {
+                    const float sum = dev::sum6(X(i - 1), X(i + 1), X(i - g.px), X(i + g.px), col[k],
+                                                col[k + 2]);
+                    r = (d[i] + (float)__popc(fl & F_NBRS) * xc) - sum;
+                    s2 += (double)r * (double)r;
+                    m = fmaxf(m, fabsf(r));
+                }
+                if (!LAST) q[i] = r;
             }
-            if (!LAST) q[i] = r;
         }
     }
     block_partials(s2, m, part, pmax);
 }
 
-// Fixed-order reduction of the partials and the control update (one block of
-// 1024 threads).  stage 0: norms of d, a new step (resets done/refinements);
-// stage > 0: the residual after a refinement.
+// First stage of the partials reduction: block b reduces the fixed chunk
+// [b*chunk, (b+1)*chunk) in a fixed order.
+__global__ __launch_bounds__(256) void k_ref_reduce(const double* __restrict__ part,
+                                                    const unsigned* __restrict__ pmax, int n,
+                                                    int chunk, double* __restrict__ part2,
+                                                    unsigned* __restrict__ pmax2,
+                                                    const RefineCtl* __restrict__ ctl, int stage)
+{
+    __shared__ double ss[256];
+    __shared__ unsigned sm[256];
+    if (stage > 0 && ctl->done) return;  // uniform
+    const int t = (int)threadIdx.x;
+    const int b0 = (int)blockIdx.x * chunk, b1 = min(b0 + chunk, n);
+    double a = 0.0;
+    unsigned m = 0;
+    for (int k = b0 + t; k < b1; k += 256) {
+        a += part[k];
+        m = max(m, pmax[k]);
+    }
+    ss[t] = a;
+    sm[t] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] += ss[t + w];
+            sm[t] = max(sm[t], sm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part2[blockIdx.x] = ss[0];
+        pmax2[blockIdx.x] = sm[0];
+    }
+}
+
+// Fixed-order reduction of the first-stage partials and the control update
+// (one block of 1024 threads).  stage 0: norms of d, a new step (resets
+// done/refinements); stage > 0: the residual after a refinement.
 __global__ __launch_bounds__(1024) void k_ref_finish(RefineCtl* __restrict__ ctl,
                                                      const double* __restrict__ part,
                                                      const unsigned* __restrict__ pmax, int n,
@@ -269,28 +329,44 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
 #pragma unroll
     for (int t = 0; t <= T; ++t) iq[t] = qq[t] = zero2;
 
-    h2 ne = zero2, ni = zero2, nq = zero2;
-    auto load = [&](int k) {
-        ne = zero2;
-        ni = zero2;
-        nq = zero2;
-        if (!(in_grid && k >= 0 && k <= g.nz)) return;
+    // Prefetch ring (plane k + j in slot j), dev::kPrefetch planes deep.
+    constexpr int D = dev::kPrefetch;
+    h2 ne[D], ni[D], nq[D];
+    auto load = [&](int k, h2& e_, h2& i_, h2& q_) {
+        e_ = zero2;
+        i_ = zero2;
+        q_ = zero2;
+        if (!(in_grid && k >= 0 && k <= g.nz && k < z1 + T)) return;
         const int wi = wbase + k * wplane;
-        ni = ld_h2(inv, wi);
+        i_ = ld_h2(inv, wi);
         if (FIRST) {
             const float2 rq = *reinterpret_cast<const float2*>(q + 2 * wi);
-            nq.x = (_Float16)(rq.x * s * (float)ni.x);
-            nq.y = (_Float16)(rq.y * s * (float)ni.y);
-            if (writes && k >= z0 && k < z1) st_h2(qi, wi, nq);
+            q_.x = (_Float16)(rq.x * s * (float)i_.x);
+            q_.y = (_Float16)(rq.y * s * (float)i_.y);
+            if (writes && k >= z0 && k < z1) st_h2(qi, wi, q_);
         } else {
-            ne = ld_h2(ein, wi);
-            nq = ld_h2(qi, wi);
+            e_ = ld_h2(ein, wi);
+            q_ = ld_h2(qi, wi);
         }
     };
-    load(z0 - T);
-    for (int k = z0 - T; k < z1 + T; ++k) {
-        const h2 p0 = ne, ii = ni, qv = nq;
-        if (k + 1 < z1 + T) load(k + 1);
+#pragma unroll
+    for (int j = 0; j < D; ++j) load(z0 - T + j, ne[j], ni[j], nq[j]);
+    // Unrolled by lcm(3, T+1) so the register windows rotate without copies;
+    // the padded iterations load and write nothing.
+    constexpr int U = dev::z_unroll(T);
+    const int kend = z0 - T + ((z1 - z0 + 2 * T + U - 1) / U) * U;
+    for (int kb = z0 - T; kb < kend; kb += U)
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = kb + u;
+        const h2 p0 = ne[0], ii = ni[0], qv = nq[0];
+#pragma unroll
+        for (int j = 0; j + 1 < D; ++j) {
+            ne[j] = ne[j + 1];
+            ni[j] = ni[j + 1];
+            nq[j] = nq[j + 1];
+        }
+        load(k + D, ne[D - 1], ni[D - 1], nq[D - 1]);
 #pragma unroll
         for (int t = T; t > 0; --t) {
             iq[t] = iq[t - 1];
@@ -306,14 +382,19 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
         newv[0] = p0;
 #pragma unroll
         for (int t = 1; t <= T; ++t) {
-            const h2 Lp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 1][tx]);
-            const h2 Rp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 1][tx + 2]);
+            // x neighbours from the adjacent lanes (DPP), y neighbours from LDS.
+            const h2 C = w[t - 1][1];
+            const uint32_t cw = __builtin_bit_cast(uint32_t, C);
+            const h2 Lp = __builtin_bit_cast(h2, dev::lane_prev(cw));
+            const h2 Rp = __builtin_bit_cast(h2, dev::lane_next(cw));
             const h2 Up = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty][tx + 1]);
             const h2 Dp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 2][tx + 1]);
-            const h2 C = w[t - 1][1];
             const h2 xm = {Lp.y, C.x};
             const h2 xp = {C.y, Rp.x};
-            const h2 sum = ((((xm + xp) + Up) + Dp) + w[t - 1][0]) + w[t - 1][2];
+            // Tree order, the z+ value (level t-1 of this iteration) last:
+            // the chain through the levels is one add and one fma per level.
+            const h2 part = ((xm + xp) + (Up + Dp)) + w[t - 1][0];
+            const h2 sum = part + w[t - 1][2];
             const h2 v = __builtin_elementwise_fma(iq[t], sum, -qq[t]);
             newv[t] = v;
             if (t < T) {
@@ -331,14 +412,28 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
     }
 }
 
-inline dim3 resid_grid(const Grid& g, int* zper)
+inline dim3 resid_grid(const Grid& g)
 {
-    const int gx = (g.nx - 2 + BX - 1) / BX, gy = (g.ny - 2 + BY - 1) / BY;
-    int zs = kRefineParts / (gx * gy);
-    zs = zs < 1 ? 1 : (zs > g.nz - 2 ? g.nz - 2 : zs);
-    *zper = (g.nz - 2 + zs - 1) / zs;
-    zs = (g.nz - 2 + *zper - 1) / *zper;
-    return dim3((unsigned)gx, (unsigned)gy, (unsigned)zs);
+    return dim3((unsigned)((g.nx - 2 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
+                (unsigned)((g.nz - 2 + kZB - 1) / kZB));
+}
+
+// Two-stage fixed-order reduction of n partials (second stage in the
+// finish kernel), then the control update.
+hipError_t reduce_and_finish(const RefineBufs& b, int n, int stage, double tol, hipStream_t st)
+{
+    const int chunk = (n + kRefineStage2 - 1) / kRefineStage2;
+    const int na = (n + chunk - 1) / chunk;
+    double* part2 = b.part + n;
+    unsigned* pmax2 = b.pmax + n;
+    hipLaunchKernelGGL(k_ref_reduce, dim3((unsigned)na), dim3(256), 0, st, (const double*)b.part,
+                       (const unsigned*)b.pmax, n, chunk, part2, pmax2, (const RefineCtl*)b.ctl,
+                       stage);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(1024), 0, st, b.ctl, (const double*)part2,
+                       (const unsigned*)pmax2, na, stage, tol);
+    return hipGetLastError();
 }
 
 template <int T, bool FIRST>
@@ -369,22 +464,18 @@ hipError_t refine_build_inv(const Grid& g, const uint8_t* flags, const RefineBuf
 
 int refine_parts(const Grid& g)
 {
-    int zper = 0;
-    dim3 gr = resid_grid(g, &zper);
+    const dim3 gr = resid_grid(g);
     return (int)(gr.x * gr.y * gr.z);
 }
 
 hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, const RefineBufs& b,
                         double tol, hipStream_t st)
 {
-    int zper = 0;
-    const dim3 grid = resid_grid(g, &zper);
-    hipLaunchKernelGGL(k_ref_norm0, grid, dim3(BX, BY), 0, st, g, d, flags, b.part, b.pmax, zper);
+    const dim3 grid = resid_grid(g);
+    hipLaunchKernelGGL(k_ref_norm0, grid, dim3(BX, BY), 0, st, g, d, flags, b.part, b.pmax);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(1024), 0, st, b.ctl, (const double*)b.part,
-                       (const unsigned*)b.pmax, (int)(grid.x * grid.y * grid.z), 0, tol);
-    return hipGetLastError();
+    return reduce_and_finish(b, (int)(grid.x * grid.y * grid.z), 0, tol, st);
 }
 
 hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, int first, int ein,
@@ -406,26 +497,23 @@ hipError_t refine_resid(const Grid& g, const float* xo, float* xn, const float* 
                         const uint8_t* flags, const RefineBufs& b, int ein, int xzero, int last,
                         int stage, double tol, hipStream_t st)
 {
-    int zper = 0;
-    const dim3 grid = resid_grid(g, &zper);
+    const dim3 grid = resid_grid(g);
     const uint16_t* e = reinterpret_cast<const uint16_t*>(b.e[ein]);
     if (xzero && last)
         hipLaunchKernelGGL((k_ref_resid<true, true>), grid, dim3(BX, BY), 0, st, g, xo, xn, d, flags,
-                           e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+                           e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax);
     else if (xzero)
         hipLaunchKernelGGL((k_ref_resid<true, false>), grid, dim3(BX, BY), 0, st, g, xo, xn, d,
-                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax);
     else if (last)
         hipLaunchKernelGGL((k_ref_resid<false, true>), grid, dim3(BX, BY), 0, st, g, xo, xn, d,
-                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax);
     else
         hipLaunchKernelGGL((k_ref_resid<false, false>), grid, dim3(BX, BY), 0, st, g, xo, xn, d,
-                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax, zper);
+                           flags, e, b.q, (const RefineCtl*)b.ctl, b.part, b.pmax);
     hipError_t e2 = hipGetLastError();
     if (e2 != hipSuccess) return e2;
-    hipLaunchKernelGGL(k_ref_finish, dim3(1), dim3(1024), 0, st, b.ctl, (const double*)b.part,
-                       (const unsigned*)b.pmax, (int)(grid.x * grid.y * grid.z), stage, tol);
-    return hipGetLastError();
+    return reduce_and_finish(b, (int)(grid.x * grid.y * grid.z), stage, tol, st);
 }
 
 }  // namespace smk

@@ -79,7 +79,7 @@ size_t arena_bytes(const Grid& g, int storage)
 size_t refine_bytes(const Grid& g)
 {
     size_t e = (size_t)g.elems;
-    size_t parts = (size_t)smk::refine_parts(g);
+    size_t parts = (size_t)smk::refine_parts(g) + smk::kRefineStage2;
     return align256(e * sizeof(float)) + 4 * align256(e * 2) + align256(sizeof(smk::RefineCtl)) +
            align256(parts * sizeof(double)) + align256(parts * sizeof(unsigned));
 }
@@ -194,7 +194,7 @@ smoke_status ensure_refine(smoke_sim* s)
     b.ctl = reinterpret_cast<smk::RefineCtl*>(p);
     p += align256(sizeof(smk::RefineCtl));
     b.part = reinterpret_cast<double*>(p);
-    p += align256((size_t)smk::refine_parts(g) * sizeof(double));
+    p += align256(((size_t)smk::refine_parts(g) + smk::kRefineStage2) * sizeof(double));
     b.pmax = reinterpret_cast<unsigned*>(p);
     s->refine_mem = m;
     hipError_t err = smk::fill_u8(reinterpret_cast<uint8_t*>(m), (long long)bytes, 0, s->stream);

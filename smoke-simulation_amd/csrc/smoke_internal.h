@@ -234,9 +234,10 @@ struct RefineBufs {
     double* part = nullptr;        // per-block FP64 partial sums of r^2
     unsigned* pmax = nullptr;      // per-block max |r| (FP32 bits)
 };
-// Target number of blocks of a residual pass (one partial each).
-constexpr int kRefineParts = 4096;
-// Blocks (partials) of a residual pass over g.
+// First-stage partials of the two-stage norm reduction (at most).
+constexpr int kRefineStage2 = 1024;
+// Blocks (partials) of a residual pass over g; the partial buffers hold
+// refine_parts(g) + kRefineStage2 entries.
 int refine_parts(const Grid& g);
 hipError_t refine_build_inv(const Grid& g, const uint8_t* flags, const RefineBufs& b,
                             hipStream_t st);

@@ -77,20 +77,48 @@ __device__ __forceinline__ void source_at(const Grid& g, T* __restrict__ r0, T* 
 // integrate (smokeSimulation.cu:315-329) and velocityConfinement (:331-352)
 // fused per staggered index: the buoyancy range is x in [0,nx-1],
 // y in [1,ny-1], z in [0,nz-1]; the clamp range x,y,z in [1,n-1] lies inside
-// it and sees the post-buoyancy v of the same index.
+// it and sees the post-buoyancy v of the same index.  All loads of the
+// per-cell bodies are issued unconditionally (every address is inside the
+// padded field for the cells the range checks admit), so they do not wait
+// for the flags; only the stores are predicated.
+//
+// Each body is split into a load phase and an apply phase; the *_col
+// versions run the load phase of NZ cells of one (x, y) column first, so a
+// thread keeps all their loads in flight, then apply them in z order (the
+// cells of a stage are independent).  *_at is the one-cell case.
+struct ForceIn {
+    float v, r, u, w;
+    uint8_t fl;
+    bool ok;
+};
 template <typename T>
-__device__ __forceinline__ void force_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
-                                         T* __restrict__ w, const T* __restrict__ rho,
-                                         const uint8_t* __restrict__ flags, const ForceArgs& a,
-                                         int x, int y, int z)
+__device__ __forceinline__ ForceIn force_load(const Grid& g, const T* __restrict__ u,
+                                              const T* __restrict__ v, const T* __restrict__ w,
+                                              const T* __restrict__ rho,
+                                              const uint8_t* __restrict__ flags, int x, int y, int z)
 {
-    if (x >= g.nx || y < 1 || y >= g.ny || z >= g.nz) return;
-    int i = g.idx(x, y, z);
-    uint8_t fl = flags[i];
-    float vv = ldf(v, i);
+    ForceIn in{0.0f, 0.0f, 0.0f, 0.0f, 0, false};
+    if (x >= g.nx || y < 1 || y >= g.ny || z >= g.nz) return in;
+    const int i = g.idx(x, y, z);
+    in.ok = true;
+    in.fl = flags[i];
+    in.v = ldf(v, i);
+    in.r = ldf(rho, i);
+    in.u = ldf(u, i);
+    in.w = ldf(w, i);
+    return in;
+}
+template <typename T>
+__device__ __forceinline__ void force_apply(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                            T* __restrict__ w, const ForceArgs& a,
+                                            const ForceIn& in, int x, int y, int z)
+{
+    if (!in.ok) return;
+    const int i = g.idx(x, y, z);
+    const float r = in.r, uu = in.u, ww = in.w;
+    float vv = in.v;
     bool write_v = false;
-    if ((fl & F_SELF) && (fl & F_YM)) {
-        float r = ldf(rho, i);
+    if ((in.fl & F_SELF) && (in.fl & F_YM)) {
         if (!a.gate_on || r > a.gate) {
             float b = a.buoyancy * (r - 0.0f);
             vv = vv + (r * a.gravity * a.dt + b * a.dt);
@@ -98,8 +126,6 @@ __device__ __forceinline__ void force_at(const Grid& g, T* __restrict__ u, T* __
         }
     }
     if (x >= 1 && z >= 1) {
-        float uu = ldf(u, i);
-        float ww = ldf(w, i);
         float L = uu * uu + vv * vv + ww * ww;
         if (L * a.dt > a.max_v2) {
             stf(u, i, uu * (a.max_v2 / (L * a.dt)));
@@ -109,6 +135,26 @@ __device__ __forceinline__ void force_at(const Grid& g, T* __restrict__ u, T* __
         }
     }
     if (write_v) stf(v, i, vv);
+}
+template <int NZ, typename T>
+__device__ __forceinline__ void force_col(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                          T* __restrict__ w, const T* __restrict__ rho,
+                                          const uint8_t* __restrict__ flags, const ForceArgs& a,
+                                          int x, int y, int z0)
+{
+    ForceIn in[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) in[j] = force_load(g, u, v, w, rho, flags, x, y, z0 + j);
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) force_apply(g, u, v, w, a, in[j], x, y, z0 + j);
+}
+template <typename T>
+__device__ __forceinline__ void force_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                         T* __restrict__ w, const T* __restrict__ rho,
+                                         const uint8_t* __restrict__ flags, const ForceArgs& a,
+                                         int x, int y, int z)
+{
+    force_col<1>(g, u, v, w, rho, flags, a, x, y, z);
 }
 
 // Colour-o cells of row (y,z): x = 2t + 1 + ((y+z+o+1) & 1) enumerates the
@@ -120,19 +166,41 @@ __device__ __forceinline__ int colour_x(int t, int y, int z, int o)
 
 // Right-hand side of the pressure form: d = div(u*) on active cells, in the
 // reference's operand order (smokeSimulation.cu:379-381); 0 elsewhere.
+template <int NZ, typename T>
+__device__ __forceinline__ void divergence_col(const Grid& g, const T* __restrict__ u,
+                                               const T* __restrict__ v, const T* __restrict__ w,
+                                               const uint8_t* __restrict__ flags,
+                                               float* __restrict__ d, int x, int y, int z0)
+{
+    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2) return;
+    float dv[NZ];
+    bool act[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        const int z = z0 + j;
+        dv[j] = 0.0f;
+        act[j] = false;
+        if (z < 1 || z > g.nz - 2) continue;
+        const int i = g.idx(x, y, z);
+        const float u0 = ldf(u, i), u1 = ldf(u, i + 1), v0 = ldf(v, i), v1 = ldf(v, i + g.px),
+                    w0 = ldf(w, i), w1 = ldf(w, i + g.plane);
+        act[j] = (flags[i] & F_ACTIVE) != 0;
+        dv[j] = -u0 + u1 + -v0 + v1 + -w0 + w1;
+    }
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        const int z = z0 + j;
+        if (z < 1 || z > g.nz - 2) continue;
+        d[g.idx(x, y, z)] = act[j] ? dv[j] : 0.0f;
+    }
+}
 template <typename T>
 __device__ __forceinline__ void divergence_at(const Grid& g, const T* __restrict__ u,
                                               const T* __restrict__ v, const T* __restrict__ w,
                                               const uint8_t* __restrict__ flags,
                                               float* __restrict__ d, int x, int y, int z)
 {
-    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2) return;
-    int i = g.idx(x, y, z);
-    float dv = 0.0f;
-    if (flags[i] & F_ACTIVE)
-        dv = -ldf(u, i) + ldf(u, i + 1) + -ldf(v, i) + ldf(v, i + g.px) + -ldf(w, i) +
-             ldf(w, i + g.plane);
-    d[i] = dv;
+    divergence_col<1>(g, u, v, w, flags, d, x, y, z);
 }
 
 __device__ __forceinline__ float nbr_sum(const float* __restrict__ p, int i, uint8_t fl, int px,
@@ -205,6 +273,38 @@ __device__ __forceinline__ void jacobi_at(const Grid& g, const float* __restrict
     pout[i] = (inv > 0.0f) ? out : 0.0f;
 }
 
+// Unroll factor of the z-streaming loops of the fused sweeps: the register
+// windows rotate with period 3 (z neighbours) and T+1 (per-plane diagonal
+// and right-hand side), so after lcm(3, T+1) iterations every register is
+// back in its place and the unrolled body needs no copies.
+__host__ __device__ constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+__host__ __device__ constexpr int z_unroll(int T) { return 3 * (T + 1) / gcd_c(3, T + 1); }
+// Planes in flight per thread of the z-streaming loops (a divisor of every
+// z_unroll, so the prefetch ring rotates without copies as well).
+constexpr int kPrefetch = 3;
+
+// The value held by the previous / next lane of the wave64 (DPP wave_shr:1 /
+// wave_shl:1 row moves); lane 0 / lane 63 receive 0.  The fused sweeps take
+// their x neighbours this way instead of through LDS: a wave covers whole
+// tile rows, and the lanes at a row's ends are halo columns, whose values
+// never reach the written region.
+__device__ __forceinline__ uint32_t lane_prev(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ uint32_t lane_next(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float lane_prev(float v)
+{
+    return __uint_as_float(lane_prev(__float_as_uint(v)));
+}
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __uint_as_float(lane_next(__float_as_uint(v)));
+}
+
 // LDS of one fused-Jacobi block: per level, two planes of the grown tile,
 // padded by one on each side (edge threads read the never-written pad, which
 // only reaches halo columns that are not written out).
@@ -222,9 +322,10 @@ struct FusedLds {
 // k-t-1, k-t, k-t+1 kept in registers and from the four x/y neighbours of
 // level t-1 at plane k-t held in LDS (double buffered, one barrier per plane).
 // Every value is computed with the single-sweep arithmetic, so results are
-// bit-identical to T single sweeps.  The next plane is prefetched into
-// registers one iteration ahead.  All threads of the block must call this
-// with the same tile (the barriers are block-wide).
+// bit-identical to T single sweeps.  Planes are prefetched into registers
+// kPrefetch iterations ahead (one iteration is shorter than the HBM latency).
+// All threads of the block must call this with the same tile (the barriers
+// are block-wide).
 template <int TX, int TY, int T, bool DAMPED>
 __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __restrict__ pin,
                                                   float* __restrict__ pout,
@@ -251,32 +352,40 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
 #pragma unroll
     for (int t = 0; t <= T; ++t) iq[t] = dq[t] = 0.0f;
 
-    float np = 0.0f, nd = 0.0f;
-    uint8_t nf = 0;
-    {
-        const int k = z0 - T;
-        if (in_grid && k >= 0 && k < g.nz) {
-            const int i = base + k * g.plane;
-            np = zero_in ? 0.0f : pin[i];
-            nf = flags[i];
-            nd = d[i];
+    // Prefetch ring: plane k + j in slot j.
+    float np[kPrefetch], nd[kPrefetch];
+    uint8_t nf[kPrefetch];
+    auto load = [&](int kn, float& p, float& dv, uint8_t& f) {
+        p = 0.0f;
+        dv = 0.0f;
+        f = 0;
+        if (in_grid && kn >= 0 && kn < g.nz && kn < z1 + T) {
+            const int i = base + kn * g.plane;
+            p = zero_in ? 0.0f : pin[i];
+            f = flags[i];
+            dv = d[i];
         }
-    }
-    for (int k = z0 - T; k < z1 + T; ++k) {
-        const float p0 = np, dd = nd;
-        const uint8_t fl = nf;
-        {
-            const int kn = k + 1;
-            np = 0.0f;
-            nd = 0.0f;
-            nf = 0;
-            if (in_grid && kn >= 0 && kn < g.nz && kn < z1 + T) {
-                const int i = base + kn * g.plane;
-                np = zero_in ? 0.0f : pin[i];
-                nf = flags[i];
-                nd = d[i];
-            }
+    };
+#pragma unroll
+    for (int j = 0; j < kPrefetch; ++j) load(z0 - T + j, np[j], nd[j], nf[j]);
+    // The trip count is rounded up to a multiple of the unroll factor; the
+    // extra iterations load nothing (kn >= z1 + T) and write nothing
+    // (zo >= z1).
+    constexpr int U = z_unroll(T);
+    const int kend = z0 - T + ((z1 - z0 + 2 * T + U - 1) / U) * U;
+    for (int kb = z0 - T; kb < kend; kb += U)
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = kb + u;
+        const float p0 = np[0], dd = nd[0];
+        const uint8_t fl = nf[0];
+#pragma unroll
+        for (int j = 0; j + 1 < kPrefetch; ++j) {
+            np[j] = np[j + 1];
+            nd[j] = nd[j + 1];
+            nf[j] = nf[j + 1];
         }
+        load(k + kPrefetch, np[kPrefetch - 1], nd[kPrefetch - 1], nf[kPrefetch - 1]);
 #pragma unroll
         for (int t = T; t > 0; --t) {
             iq[t] = iq[t - 1];
@@ -292,7 +401,9 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
         newv[0] = p0;
 #pragma unroll
         for (int t = 1; t <= T; ++t) {
-            const float sum = sum6(sh.v[t - 1][prev][ty + 1][tx], sh.v[t - 1][prev][ty + 1][tx + 2],
+            // x neighbours of level t-1 at plane k-t from the adjacent lanes
+            // (w[t-1][1] holds that plane for every lane), y neighbours from LDS.
+            const float sum = sum6(lane_prev(w[t - 1][1]), lane_next(w[t - 1][1]),
                                    sh.v[t - 1][prev][ty][tx + 1], sh.v[t - 1][prev][ty + 2][tx + 1],
                                    w[t - 1][0], w[t - 1][2]);
             const float G = (sum - dq[t]) * iq[t];
@@ -321,19 +432,45 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
 
 // u -= grad P on every face whose two cells are fluid.  Faces that touch no
 // active cell see P_b - P_a = 0 and are skipped (x,y,z >= 1 suffices).
+template <int NZ, typename T>
+__device__ __forceinline__ void gradient_col(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                             T* __restrict__ w, const float* __restrict__ p,
+                                             const uint8_t* __restrict__ flags, int x, int y, int z0)
+{
+    if (x < 1 || x > g.nx - 1 || y < 1 || y > g.ny - 1) return;
+    float pc[NZ], pxm[NZ], pym[NZ], pzm[NZ], uu[NZ], vv[NZ], ww[NZ];
+    uint8_t fl[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        const int z = z0 + j;
+        fl[j] = 0;
+        pc[j] = pxm[j] = pym[j] = pzm[j] = uu[j] = vv[j] = ww[j] = 0.0f;
+        if (z < 1 || z > g.nz - 1) continue;
+        const int i = g.idx(x, y, z);
+        fl[j] = flags[i];
+        pc[j] = p[i];
+        pxm[j] = p[i - 1];
+        pym[j] = p[i - g.px];
+        pzm[j] = p[i - g.plane];
+        uu[j] = ldf(u, i);
+        vv[j] = ldf(v, i);
+        ww[j] = ldf(w, i);
+    }
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        if (!(fl[j] & F_SELF)) continue;  // also every cell outside the z range
+        const int i = g.idx(x, y, z0 + j);
+        if (fl[j] & F_XM) stf(u, i, uu[j] - (pc[j] - pxm[j]));
+        if (fl[j] & F_YM) stf(v, i, vv[j] - (pc[j] - pym[j]));
+        if (fl[j] & F_ZM) stf(w, i, ww[j] - (pc[j] - pzm[j]));
+    }
+}
 template <typename T>
 __device__ __forceinline__ void gradient_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
                                             T* __restrict__ w, const float* __restrict__ p,
                                             const uint8_t* __restrict__ flags, int x, int y, int z)
 {
-    if (x < 1 || x > g.nx - 1 || y < 1 || y > g.ny - 1 || z < 1 || z > g.nz - 1) return;
-    int i = g.idx(x, y, z);
-    uint8_t fl = flags[i];
-    if (!(fl & F_SELF)) return;
-    float pc = p[i];
-    if (fl & F_XM) stf(u, i, ldf(u, i) - (pc - p[i - 1]));
-    if (fl & F_YM) stf(v, i, ldf(v, i) - (pc - p[i - g.px]));
-    if (fl & F_ZM) stf(w, i, ldf(w, i) - (pc - p[i - g.plane]));
+    gradient_col<1>(g, u, v, w, p, flags, x, y, z);
 }
 
 // avgU/avgV/avgW (smokeSimulation.cu:409-447): 8-point means in the
@@ -408,7 +545,9 @@ __device__ __forceinline__ float2 ld2(const float* __restrict__ a, int r)
 }
 __device__ __forceinline__ float2 ld2(const bf16s* __restrict__ a, int r)
 {
-    return make_float2(ldf(a, r), ldf(a, r + 1));
+    uint32_t w;  // both values with one 4-byte load (2-byte aligned)
+    __builtin_memcpy(&w, a + r, sizeof w);
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
 }
 
 // The eight corners in the reference's order (:473-481).
@@ -474,6 +613,52 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const T* __res
 
 // advectSmoke (smokeSimulation.cu:617-638): cell velocity from the
 // post-advection faces, back-trace in double rounded to float.
+// The sample's corners are clamped into the field, so it is read
+// unconditionally too (a NaN position clamps to a valid cell as well); the
+// column version issues the face loads of all NZ cells, then the corner loads
+// of all NZ samples, then stores.
+template <int NZ, typename T>
+__device__ __forceinline__ void advect_density_col(const Grid& g, const T* __restrict__ r0,
+                                                   T* __restrict__ r1, const T* __restrict__ u,
+                                                   const T* __restrict__ v,
+                                                   const T* __restrict__ w,
+                                                   const uint8_t* __restrict__ flags, float dt,
+                                                   int x, int y, int z0)
+{
+    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2) return;
+    float fu0[NZ], fu1[NZ], fv0[NZ], fv1[NZ], fw0[NZ], fw1[NZ];
+    bool self[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        const int z = z0 + j;
+        self[j] = false;
+        fu0[j] = fu1[j] = fv0[j] = fv1[j] = fw0[j] = fw1[j] = 0.0f;
+        if (z < 1 || z > g.nz - 2) continue;
+        const int i = g.idx(x, y, z);
+        self[j] = (flags[i] & F_SELF) != 0;
+        fu0[j] = ldf(u, i);
+        fu1[j] = ldf(u, i + 1);
+        fv0[j] = ldf(v, i);
+        fv1[j] = ldf(v, i + g.px);
+        fw0[j] = ldf(w, i);
+        fw1[j] = ldf(w, i + g.plane);
+    }
+    float s[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+        const int z = z0 + j;
+        float ut = (fu0[j] + fu1[j]) / 2.0f;
+        float vt = (fv0[j] + fv1[j]) / 2.0f;
+        float wt = (fw0[j] + fw1[j]) / 2.0f;
+        float xt = (float)((double)x + 0.5 - (double)(ut * dt));
+        float yt = (float)((double)y + 0.5 - (double)(vt * dt));
+        float zt = (float)((double)z + 0.5 - (double)(wt * dt));
+        s[j] = tri_eval(r0, tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f));
+    }
+#pragma unroll
+    for (int j = 0; j < NZ; ++j)
+        if (self[j]) stf(r1, g.idx(x, y, z0 + j), s[j]);
+}
 template <typename T>
 __device__ __forceinline__ void advect_density_at(const Grid& g, const T* __restrict__ r0,
                                                   T* __restrict__ r1, const T* __restrict__ u,
@@ -481,16 +666,7 @@ __device__ __forceinline__ void advect_density_at(const Grid& g, const T* __rest
                                                   const uint8_t* __restrict__ flags, float dt,
                                                   int x, int y, int z)
 {
-    if (x < 1 || x > g.nx - 2 || y < 1 || y > g.ny - 2 || z < 1 || z > g.nz - 2) return;
-    int i = g.idx(x, y, z);
-    if (!(flags[i] & F_SELF)) return;
-    float ut = (ldf(u, i) + ldf(u, i + 1)) / 2.0f;
-    float vt = (ldf(v, i) + ldf(v, i + g.px)) / 2.0f;
-    float wt = (ldf(w, i) + ldf(w, i + g.plane)) / 2.0f;
-    float xt = (float)((double)x + 0.5 - (double)(ut * dt));
-    float yt = (float)((double)y + 0.5 - (double)(vt * dt));
-    float zt = (float)((double)z + 0.5 - (double)(wt * dt));
-    stf(r1, i, tri_eval(r0, tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f)));
+    advect_density_col<1>(g, r0, r1, u, v, w, flags, dt, x, y, z);
 }
 
 }  // namespace dev

@@ -6,8 +6,8 @@
 // the kernels here map threads to cells.  Thread blocks are 64 x 4 x 1: one
 // wave64 covers 64 consecutive x of one row, so every field access of a wave
 // is one contiguous 256-byte run (FP32) of a 128-byte-aligned row; blockIdx.z
-// walks z planes.  Compiled with -ffp-contract=off: results are bit-identical
-// to the oracle's FP32 mode.
+// walks z planes (kZ planes per thread in the streaming stages).  Compiled
+// with -ffp-contract=off: results are bit-identical to the oracle's FP32 mode.
 #include "step_device.h"
 
 namespace smk {
@@ -20,6 +20,16 @@ using dev::BY;
 inline dim3 grid_for(int wx, int wy, int wz)
 {
     return dim3((unsigned)((wx + BX - 1) / BX), (unsigned)((wy + BY - 1) / BY), (unsigned)wz);
+}
+
+// Cells per thread of the streaming stages (force, divergence, gradient,
+// density advection): kZ consecutive planes of one column, so each thread
+// has kZ cells' loads in flight (the *_col bodies of step_device.h).
+constexpr int kZ = 4;
+inline dim3 grid_col(int wx, int wy, int wz)
+{
+    return dim3((unsigned)((wx + BX - 1) / BX), (unsigned)((wy + BY - 1) / BY),
+                (unsigned)((wz + kZ - 1) / kZ));
 }
 
 // ------------------------------------------------------------ utilities
@@ -178,8 +188,8 @@ template <typename T>
 __global__ void k_force(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
                         const T* __restrict__ rho, const uint8_t* __restrict__ flags, ForceArgs a)
 {
-    dev::force_at(g, u, v, w, rho, flags, a, blockIdx.x * BX + threadIdx.x,
-                  blockIdx.y * BY + threadIdx.y + 1, blockIdx.z);
+    dev::force_col<kZ>(g, u, v, w, rho, flags, a, blockIdx.x * BX + threadIdx.x,
+                       blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ);
 }
 
 // divergence (smokeSimulation.cu:356-394): one red-black SOR half-sweep in
@@ -217,8 +227,8 @@ __global__ void k_divergence(Grid g, const T* __restrict__ u, const T* __restric
                              const T* __restrict__ w, const uint8_t* __restrict__ flags,
                              float* __restrict__ d)
 {
-    dev::divergence_at(g, u, v, w, flags, d, blockIdx.x * BX + threadIdx.x + 1,
-                       blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
+    dev::divergence_col<kZ>(g, u, v, w, flags, d, blockIdx.x * BX + threadIdx.x + 1,
+                            blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
 __global__ void k_jacobi(Grid g, const float* __restrict__ pin, float* __restrict__ pout,
@@ -282,8 +292,8 @@ template <typename T>
 __global__ void k_gradient(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
                            const float* __restrict__ p, const uint8_t* __restrict__ flags)
 {
-    dev::gradient_at(g, u, v, w, p, flags, blockIdx.x * BX + threadIdx.x + 1,
-                     blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
+    dev::gradient_col<kZ>(g, u, v, w, p, flags, blockIdx.x * BX + threadIdx.x + 1,
+                          blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
 template <typename T>
@@ -300,8 +310,8 @@ __global__ void k_advect_density(Grid g, const T* __restrict__ r0, T* __restrict
                                  const T* __restrict__ u, const T* __restrict__ v,
                                  const T* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
 {
-    dev::advect_density_at(g, r0, r1, u, v, w, flags, dt, blockIdx.x * BX + threadIdx.x + 1,
-                           blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
+    dev::advect_density_col<kZ>(g, r0, r1, u, v, w, flags, dt, blockIdx.x * BX + threadIdx.x + 1,
+                                blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
 }  // namespace
@@ -481,7 +491,7 @@ template <typename T>
 static hipError_t launch_force(const Grid& g, T* u, T* v, T* w, const T* rho, const uint8_t* flags,
                                const ForceArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_force<T>, grid_for(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
+    hipLaunchKernelGGL(k_force<T>, grid_col(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
                        rho, flags, a);
     return hipGetLastError();
 }
@@ -509,7 +519,7 @@ template <typename T>
 static hipError_t launch_divergence(const Grid& g, const T* u, const T* v, const T* w,
                                     const uint8_t* flags, float* d, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_divergence<T>, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st,
+    hipLaunchKernelGGL(k_divergence<T>, grid_col(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st,
                        g, u, v, w, flags, d);
     return hipGetLastError();
 }
@@ -602,7 +612,7 @@ template <typename T>
 static hipError_t launch_gradient(const Grid& g, T* u, T* v, T* w, const float* p,
                                   const uint8_t* flags, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gradient<T>, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st,
+    hipLaunchKernelGGL(k_gradient<T>, grid_col(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st,
                        g, u, v, w, p, flags);
     return hipGetLastError();
 }
@@ -643,7 +653,7 @@ template <typename T>
 static hipError_t launch_advect_density(const Grid& g, const T* r0, T* r1, const T* u, const T* v,
                                         const T* w, const uint8_t* flags, float dt, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advect_density<T>, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
+    hipLaunchKernelGGL(k_advect_density<T>, grid_col(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
                        st, g, r0, r1, u, v, w, flags, dt);
     return hipGetLastError();
 }

@@ -127,6 +127,23 @@ void emu_syncthreads()
     g_barrier->arrive_and_wait();
 }
 
+int emu_dpp(int old, int src, int ctrl, bool bound_ctrl)
+{
+    if (!g_threaded) die("DPP in a kernel run without threads");
+    const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    const int lane = t & 63;
+    int from;
+    if (ctrl == 0x138) from = lane - 1;       // wave_shr:1
+    else if (ctrl == 0x130) from = lane + 1;  // wave_shl:1
+    else die("unsupported DPP control");
+    g_exchange[t] = (unsigned)src;
+    g_barrier->arrive_and_wait();
+    const int r = (from >= 0 && from < 64) ? (int)(unsigned)g_exchange[t - lane + from]
+                                           : (bound_ctrl ? 0 : old);
+    g_barrier->arrive_and_wait();
+    return r;
+}
+
 unsigned long long emu_exchange(unsigned long long bits, int partner)
 {
     if (!g_threaded) die("lane exchange in a kernel run without threads");

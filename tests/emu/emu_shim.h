@@ -42,6 +42,12 @@ inline float __uint_as_float(unsigned u)
 // Lane exchange of a block run on host threads: every thread of the block
 // publishes its value and reads its partner's (both sides of a barrier).
 unsigned long long emu_exchange(unsigned long long bits, int partner);
+
+// DPP row moves used by the kernels: wave_shr:1 (0x138) and wave_shl:1 (0x130)
+// within each wave64; a lane without a source gets 0 (bound_ctrl) or `old`.
+int emu_dpp(int old, int src, int ctrl, bool bound_ctrl);
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, row_mask, bank_mask, bound_ctrl) \
+    emu_dpp((old), (src), (ctrl), (bound_ctrl))
 template <typename T>
 T __shfl_xor(T v, int mask, int width = 64)
 {
