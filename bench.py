@@ -133,8 +133,18 @@ def error_check(sc, steps, mixed_iterations):
     import oracle as orc
     from smokesim import scenes
     nx, ny, nz = sc["dims"]
+    # The FP32 form and the oracle run the Jacobi projection (the oracle has
+    # no mixed-precision solver parameters).
+    sc = dict(sc, params={k: v for k, v in sc["params"].items()
+                          if k not in ("inner_sweeps", "tolerance")})
+    sc["params"]["projection"] = 2
+    # The mixed form and its two halves: BF16 fields alone (FP32 Jacobi) and
+    # the FP16-inner refined solve alone (FP32 fields).
+    refined = dict(MIXED, iterations=mixed_iterations)
     forms = {"fp32": (smokesim.STORAGE_FP32, {}),
-             "mixed": (smokesim.STORAGE_BF16, dict(MIXED, iterations=mixed_iterations))}
+             "mixed": (smokesim.STORAGE_BF16, refined),
+             "bf16_fields_fp32_jacobi": (smokesim.STORAGE_BF16, {}),
+             "fp32_fields_refined": (smokesim.STORAGE_FP32, refined)}
     got = {}
     for name, (storage, extra) in forms.items():
         s = smokesim.Simulation(nx, ny, nz, storage=storage)

@@ -146,6 +146,16 @@ size_t smoke_footprint_bytes_ex(unsigned nx, unsigned ny, unsigned nz, int stora
  * solve and the refinements it ran.  Waits for the simulation's work. */
 smoke_status smoke_get_solve_info(smoke_sim* sim, double* rel_residual, int* refinements);
 
+/* Checkpoint / resume (no reference counterpart: the reference's state lives
+ * in module globals).  A snapshot is a host blob holding every device field
+ * of both ping-pong halves, the masks, the ping-pong parities, the parameters,
+ * the Jacobi plan and the object list; restoring it into a simulation of the
+ * same dims and storage continues bit for bit as the original would.  Both
+ * wait for the simulation's work. */
+size_t smoke_snapshot_bytes(const smoke_sim* sim);
+smoke_status smoke_snapshot(smoke_sim* sim, void* host, size_t bytes);
+smoke_status smoke_restore(smoke_sim* sim, const void* host, size_t bytes);
+
 /* Replaces the globals behind getGravity/getBuoyancy (smokeSimulation.cu:28-36)
  * and the compile-time constants (:38-39, :797). */
 smoke_status smoke_set_params(smoke_sim* sim, const smoke_params* p);

@@ -771,6 +771,102 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
     return SMOKE_OK;
 }
 
+namespace {
+
+// Snapshot blob: this header, then the object list, then the raw arena (its
+// layout is a function of dims and storage only).
+struct SnapshotHeader {
+    char magic[8];  // "SMKSNAP1"
+    uint32_t nx, ny, nz;
+    int32_t storage;
+    int32_t now, pcur;
+    int32_t n_objects;
+    int32_t plan[3];
+    uint64_t arena_bytes;
+    smoke_params prm;
+};
+
+size_t snapshot_size(const smoke_sim* s, size_t n_objects)
+{
+    return sizeof(SnapshotHeader) + n_objects * sizeof(Object) + arena_bytes(s->g, s->storage);
+}
+
+}  // namespace
+
+size_t smoke_snapshot_bytes(const smoke_sim* s)
+{
+    return s ? snapshot_size(s, s->objects.size()) : 0;
+}
+
+smoke_status smoke_snapshot(smoke_sim* s, void* host, size_t bytes)
+{
+    if (!s || !host) return fail(SMOKE_ERR_INVALID, "snapshot: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const size_t need = snapshot_size(s, s->objects.size());
+    if (bytes < need) return fail(SMOKE_ERR_INVALID, "snapshot: buffer holds %zu bytes, needs %zu", bytes, need);
+    SMOKE_WAIT_BATCH(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    SnapshotHeader h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, "SMKSNAP1", 8);
+    h.nx = (uint32_t)s->g.nx;
+    h.ny = (uint32_t)s->g.ny;
+    h.nz = (uint32_t)s->g.nz;
+    h.storage = s->storage;
+    h.now = s->now;
+    h.pcur = s->pcur;
+    h.n_objects = (int32_t)s->objects.size();
+    h.plan[0] = s->plan.sweeps_per_launch;
+    h.plan[1] = s->plan.tile;
+    h.plan[2] = s->plan.zc;
+    h.arena_bytes = arena_bytes(s->g, s->storage);
+    h.prm = s->prm;
+    char* p = static_cast<char*>(host);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    if (!s->objects.empty()) std::memcpy(p, s->objects.data(), s->objects.size() * sizeof(Object));
+    p += s->objects.size() * sizeof(Object);
+    HIP_TRY(hipMemcpy(p, s->arena, h.arena_bytes, hipMemcpyDeviceToHost));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
+{
+    if (!s || !host) return fail(SMOKE_ERR_INVALID, "restore: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "restore");
+    SnapshotHeader h;
+    if (bytes < sizeof h) return fail(SMOKE_ERR_INVALID, "restore: blob too short");
+    std::memcpy(&h, host, sizeof h);
+    if (std::memcmp(h.magic, "SMKSNAP1", 8) != 0) return fail(SMOKE_ERR_INVALID, "restore: not a snapshot");
+    if ((int)h.nx != s->g.nx || (int)h.ny != s->g.ny || (int)h.nz != s->g.nz || h.storage != s->storage)
+        return fail(SMOKE_ERR_INVALID, "restore: snapshot of a %ux%ux%u storage-%d simulation, this one is %dx%dx%d storage-%d",
+                    h.nx, h.ny, h.nz, h.storage, s->g.nx, s->g.ny, s->g.nz, s->storage);
+    if (h.n_objects < 0 || h.n_objects > 2 * SMOKE_MAX_OBJECTS || h.arena_bytes != arena_bytes(s->g, s->storage) ||
+        bytes < snapshot_size(s, (size_t)h.n_objects) || (h.now != 0 && h.now != 1) ||
+        (h.pcur != 0 && h.pcur != 1) || h.plan[0] < 1 || h.plan[0] > 6 ||
+        (h.plan[1] != SMOKE_TILE_32x32 && h.plan[1] != SMOKE_TILE_64x16) || h.plan[2] < 1)
+        return fail(SMOKE_ERR_INVALID, "restore: inconsistent snapshot");
+    smoke_status st = check_params(&h.prm);
+    if (st != SMOKE_OK) return st;
+    if (h.prm.projection == SMOKE_PROJ_REFINED && (st = ensure_refine(s)) != SMOKE_OK) return st;
+    const char* p = static_cast<const char*>(host) + sizeof h;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->objects.assign(reinterpret_cast<const Object*>(p), reinterpret_cast<const Object*>(p) + h.n_objects);
+    p += (size_t)h.n_objects * sizeof(Object);
+    HIP_TRY(hipMemcpy(s->arena, p, h.arena_bytes, hipMemcpyHostToDevice));
+    s->now = h.now;
+    s->pcur = h.pcur;
+    s->prm = h.prm;
+    s->plan.sweeps_per_launch = h.plan[0];
+    s->plan.tile = h.plan[1];
+    s->plan.zc = h.plan[2];
+    s->mask_dirty = true;  // flags (and the refined solver's inverse) follow the restored masks
+    HIP_TRY(refresh_mask(s, nullptr));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
 smoke_status smoke_get_solve_info(smoke_sim* s, double* rel_residual, int* refinements)
 {
     if (!s) return fail(SMOKE_ERR_INVALID, "get_solve_info: NULL sim");

@@ -95,6 +95,9 @@ def _declare(lib):
         "smoke_footprint_bytes": (ctypes.c_size_t, [u, u, u]),
         "smoke_footprint_bytes_ex": (ctypes.c_size_t, [u, u, u, i, i]),
         "smoke_get_solve_info": (st, [vp, dp, ip]),
+        "smoke_snapshot_bytes": (ctypes.c_size_t, [vp]),
+        "smoke_snapshot": (st, [vp, vp, ctypes.c_size_t]),
+        "smoke_restore": (st, [vp, vp, ctypes.c_size_t]),
         "smoke_set_params": (st, [vp, pp]),
         "smoke_get_params": (st, [vp, pp]),
         "smoke_set_jacobi_plan": (st, [vp, ctypes.POINTER(JacobiPlan)]),
@@ -223,6 +226,18 @@ class Simulation:
         rel, n = ctypes.c_double(), ctypes.c_int()
         _check(self.lib.smoke_get_solve_info(self.h, ctypes.byref(rel), ctypes.byref(n)))
         return rel.value, n.value
+
+    def snapshot(self):
+        """The whole simulation state as bytes (smoke_snapshot)."""
+        n = int(self.lib.smoke_snapshot_bytes(self.h))
+        buf = (ctypes.c_char * n)()
+        _check(self.lib.smoke_snapshot(self.h, buf, n))
+        return bytes(buf)
+
+    def restore(self, blob):
+        """Load a snapshot taken from a simulation of the same dims and storage."""
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        _check(self.lib.smoke_restore(self.h, buf, len(blob)))
 
     def close(self):
         h = getattr(self, "h", None)

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <barrier>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -70,7 +72,60 @@ void check(hipEvent_t e)
     if (!e || e->magic != kEvent) die("use of an invalid event");
 }
 
-unsigned long long g_exchange[1024];
+// Lane exchanges use two buffers in turn, so one barrier per exchange
+// suffices: a thread can only overwrite a buffer after every thread has
+// passed the next exchange's barrier, i.e. finished reading it.
+unsigned long long g_xbuf[2][1024];
+thread_local unsigned g_xcount = 0;
+
+// Persistent host threads for the threaded kernels (created once, reused by
+// every launch; never joined: the process exits with them parked).
+class Pool {
+public:
+    void run(unsigned n, std::function<void(unsigned)> job)
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        while (threads_.size() < n) {
+            const unsigned id = (unsigned)threads_.size();
+            threads_.emplace_back([this, id]() { worker(id); });
+            threads_.back().detach();
+        }
+        job_ = std::move(job);
+        njob_ = n;
+        active_ = n;
+        ++gen_;
+        cv_.notify_all();
+        done_.wait(lk, [&]() { return active_ == 0; });
+    }
+
+private:
+    void worker(unsigned id)
+    {
+        unsigned long long seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [&]() { return gen_ != seen; });
+            seen = gen_;
+            if (id >= njob_) continue;
+            std::function<void(unsigned)> job = job_;
+            lk.unlock();
+            job(id);
+            lk.lock();
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> threads_;
+    std::function<void(unsigned)> job_;
+    unsigned njob_ = 0, active_ = 0;
+    unsigned long long gen_ = 0;
+};
+Pool& pool()
+{
+    static Pool* p = new Pool();  // intentionally never destroyed
+    return *p;
+}
 
 void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 block)
 {
@@ -99,22 +154,19 @@ void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 bloc
     std::barrier<> bar((std::ptrdiff_t)nt);
     g_barrier = &bar;
     g_threaded = true;
-    std::vector<std::thread> threads;
-    threads.reserve(nt);
-    for (unsigned t = 0; t < nt; ++t)
-        threads.emplace_back([&, t]() {
-            blockDim = bd;
-            gridDim = gd;
-            set_thread(t);
-            for (unsigned bz = 0; bz < grid.z; ++bz)
-                for (unsigned by = 0; by < grid.y; ++by)
-                    for (unsigned bx = 0; bx < grid.x; ++bx) {
-                        blockIdx = {bx, by, bz};
-                        fn();
-                        bar.arrive_and_wait();  // the block ends before the next starts
-                    }
-        });
-    for (std::thread& t : threads) t.join();
+    pool().run(nt, [&](unsigned t) {
+        blockDim = bd;
+        gridDim = gd;
+        g_xcount = 0;
+        set_thread(t);
+        for (unsigned bz = 0; bz < grid.z; ++bz)
+            for (unsigned by = 0; by < grid.y; ++by)
+                for (unsigned bx = 0; bx < grid.x; ++bx) {
+                    blockIdx = {bx, by, bz};
+                    fn();
+                    bar.arrive_and_wait();  // the block ends before the next starts
+                }
+    });
     g_threaded = false;
     g_barrier = nullptr;
 }
@@ -127,6 +179,15 @@ void emu_syncthreads()
     g_barrier->arrive_and_wait();
 }
 
+static unsigned long long exchange(unsigned long long bits, int partner_t)
+{
+    const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    unsigned long long* buf = g_xbuf[g_xcount++ & 1];
+    buf[t] = bits;
+    g_barrier->arrive_and_wait();
+    return buf[partner_t];
+}
+
 int emu_dpp(int old, int src, int ctrl, bool bound_ctrl)
 {
     if (!g_threaded) die("DPP in a kernel run without threads");
@@ -136,23 +197,15 @@ int emu_dpp(int old, int src, int ctrl, bool bound_ctrl)
     if (ctrl == 0x138) from = lane - 1;       // wave_shr:1
     else if (ctrl == 0x130) from = lane + 1;  // wave_shl:1
     else die("unsupported DPP control");
-    g_exchange[t] = (unsigned)src;
-    g_barrier->arrive_and_wait();
-    const int r = (from >= 0 && from < 64) ? (int)(unsigned)g_exchange[t - lane + from]
-                                           : (bound_ctrl ? 0 : old);
-    g_barrier->arrive_and_wait();
-    return r;
+    const bool ok = from >= 0 && from < 64;
+    const int r = (int)(unsigned)exchange((unsigned)src, ok ? t - lane + from : t);
+    return ok ? r : (bound_ctrl ? 0 : old);
 }
 
 unsigned long long emu_exchange(unsigned long long bits, int partner)
 {
     if (!g_threaded) die("lane exchange in a kernel run without threads");
-    const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
-    g_exchange[t] = bits;
-    g_barrier->arrive_and_wait();
-    const unsigned long long r = g_exchange[partner];
-    g_barrier->arrive_and_wait();
-    return r;
+    return exchange(bits, partner);
 }
 
 hipError_t emu_last_error() { return hipSuccess; }
