@@ -86,7 +86,9 @@ typedef struct smoke_params {
     int warm_start;     /* pressure forms: start from the last step's P       */
     int inner_sweeps;   /* REFINED: FP16 sweeps per refinement (2..1000, even),
                            default 20                                         */
-    int reserved;       /* 0                                                  */
+    float vorticity;    /* vorticity-confinement strength eps (>= 0); 0 = off,
+                           the reference (README.md:33 lists it as future
+                           work)                                              */
     double tolerance;   /* REFINED: stop at this relative residual; 0: run all */
 } smoke_params;
 

@@ -33,13 +33,14 @@ class OrcParams(ctypes.Structure):
         ("iterations", ctypes.c_int),
         ("obstacle_mode", ctypes.c_int),
         ("warm_start", ctypes.c_int),
+        ("vorticity", ctypes.c_float),
     ]
 
 
 DEFAULT_PARAMS = dict(
     gravity=-9.82, buoyancy=2.0, density_gate=0.0, max_velocity=3.0,
     sor_omega=1.9, jacobi_omega=1.0, projection=PROJ_REFERENCE, iterations=30,
-    obstacle_mode=OBST_LAST_WINS, warm_start=0,
+    obstacle_mode=OBST_LAST_WINS, warm_start=0, vorticity=0.0,
 )
 
 _lib = None

@@ -81,6 +81,9 @@ enum : uint8_t {
     F_ACTIVE = 128
 };
 constexpr uint8_t F_NBRS = F_XM | F_XP | F_YM | F_YP | F_ZM | F_ZP;
+// Vorticity confinement: no force where |grad |omega|| <= this (cells/s per
+// cell^2), i.e. where the gradient is rounding noise and has no direction.
+constexpr float kVortMinGrad = 1e-6f;
 
 struct Params {
     float gravity;        // smokeSimulation.cu:28 (-9.82)
@@ -93,6 +96,7 @@ struct Params {
     int iterations;       // smokeSimulation.cu:797 (30)
     int obstacle_mode;    // ObstacleMode
     int warm_start;       // pressure forms: start from the previous P
+    float vorticity;      // vorticity-confinement strength eps; 0 = off (reference)
 };
 
 struct Sphere {
@@ -116,6 +120,7 @@ Params default_params()
     p.iterations = 30;
     p.obstacle_mode = OBST_LAST_WINS;
     p.warm_start = 0;
+    p.vorticity = 0.0f;
     return p;
 }
 
@@ -330,6 +335,83 @@ struct Sim : SimBase {
                         U[fi] = ut * (M2 / (L * dt));
                         V[fi] = vt * (M2 / (L * dt));
                         W[fi] = wt * (M2 / (L * dt));
+                    }
+                }
+        if (prm.vorticity > 0.0f) vorticity_confinement(dt);
+    }
+
+    // Vorticity confinement (listed as future work, README.md:33; no reference
+    // code, so this defines it): with cell-centred velocities c = mean of the
+    // two faces of a cell, omega = curl c by central differences on interior
+    // fluid cells (0 elsewhere), eta = grad |omega| by central differences,
+    // N = eta/|eta| (no force where |eta| <= kVortMinGrad) and F = eps * (N x omega) on
+    // interior fluid cells; every face between two fluid cells gains
+    // dt * (mean of its two cells' F).  Runs after the clamp on [now].
+    Real ucen(const std::vector<Real>& A, int x, int y, int z, int axis) const
+    {
+        const long a = f(x, y, z);
+        const long b = axis == 0 ? f(x + 1, y, z) : (axis == 1 ? f(x, y + 1, z) : f(x, y, z + 1));
+        return (A[a] + A[b]) * Real(0.5);
+    }
+    void curl_at(int x, int y, int z, Real& ox, Real& oy, Real& oz) const
+    {
+        const std::vector<Real>& U = u[now];
+        const std::vector<Real>& V = v[now];
+        const std::vector<Real>& W = w[now];
+        const Real h = Real(0.5);
+        ox = (ucen(W, x, y + 1, z, 2) - ucen(W, x, y - 1, z, 2)) * h -
+             (ucen(V, x, y, z + 1, 1) - ucen(V, x, y, z - 1, 1)) * h;
+        oy = (ucen(U, x, y, z + 1, 0) - ucen(U, x, y, z - 1, 0)) * h -
+             (ucen(W, x + 1, y, z, 2) - ucen(W, x - 1, y, z, 2)) * h;
+        oz = (ucen(V, x + 1, y, z, 1) - ucen(V, x - 1, y, z, 1)) * h -
+             (ucen(U, x, y + 1, z, 0) - ucen(U, x, y - 1, z, 0)) * h;
+    }
+    void vorticity_confinement(Real dt)
+    {
+        const Real eps = (Real)prm.vorticity;
+        std::vector<Real> mag(N, Real(0)), F[3];
+        for (auto& a : F) a.assign(N, Real(0));
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z < nz - 1; ++z)
+            for (int y = 1; y < ny - 1; ++y)
+                for (int x = 1; x < nx - 1; ++x) {
+                    if (!s[c(x, y, z)]) continue;
+                    Real ox, oy, oz;
+                    curl_at(x, y, z, ox, oy, oz);
+                    mag[c(x, y, z)] = std::sqrt(ox * ox + oy * oy + oz * oz);
+                }
+#pragma omp parallel for schedule(static)
+        for (int z = 1; z < nz - 1; ++z)
+            for (int y = 1; y < ny - 1; ++y)
+                for (int x = 1; x < nx - 1; ++x) {
+                    const long i = c(x, y, z);
+                    if (!s[i]) continue;
+                    const Real h = Real(0.5);
+                    const Real ex = (mag[i + 1] - mag[i - 1]) * h;
+                    const Real ey = (mag[i + nx] - mag[i - nx]) * h;
+                    const Real ez = (mag[i + (long)nx * ny] - mag[i - (long)nx * ny]) * h;
+                    const Real len = std::sqrt(ex * ex + ey * ey + ez * ez);
+                    // |grad|omega|| at rounding-noise level (a uniform vorticity
+                    // gives ~1e-9 in FP32) has no direction: no force there.
+                    if (!(len > Real(kVortMinGrad))) continue;
+                    const Real nxv = ex / len, nyv = ey / len, nzv = ez / len;
+                    Real ox, oy, oz;
+                    curl_at(x, y, z, ox, oy, oz);
+                    F[0][i] = eps * (nyv * oz - nzv * oy);
+                    F[1][i] = eps * (nzv * ox - nxv * oz);
+                    F[2][i] = eps * (nxv * oy - nyv * ox);
+                }
+        std::vector<Real>* vel[3] = {&u[now], &v[now], &w[now]};
+        const long step[3] = {1, nx, (long)nx * ny};
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const long i = c(x, y, z);
+                    const int pos[3] = {x, y, z};
+                    for (int k = 0; k < 3; ++k) {
+                        if (pos[k] < 1 || !s[i] || !s[i - step[k]]) continue;  // face between two fluid cells
+                        (*vel[k])[f(x, y, z)] += dt * ((F[k][i - step[k]] + F[k][i]) * Real(0.5));
                     }
                 }
     }
@@ -635,6 +717,7 @@ struct orc_params {
     float gravity, buoyancy, density_gate, max_velocity;
     double sor_omega, jacobi_omega;
     int projection, iterations, obstacle_mode, warm_start;
+    float vorticity;
 };
 
 void* orc_create(int real_bits, int nx, int ny, int nz)
@@ -678,6 +761,7 @@ void orc_set_params(void* h, const orc_params* p)
     s->prm.iterations = p->iterations;
     s->prm.obstacle_mode = p->obstacle_mode;
     s->prm.warm_start = p->warm_start;
+    s->prm.vorticity = p->vorticity;
 }
 
 // addObstacle / addSmokeSource (smokeSimulation.cu:88-104): ids are shared

@@ -108,6 +108,8 @@ smoke_status check_params(const smoke_params* p)
         if (p->iterations < 1)
             return fail(SMOKE_ERR_INVALID, "params: REFINED needs iterations >= 1");
     }
+    if (!std::isfinite(p->vorticity) || p->vorticity < 0.0f)
+        return fail(SMOKE_ERR_INVALID, "params: vorticity must be finite and >= 0");
     if (p->obstacle_mode != SMOKE_OBSTACLES_LAST_WINS && p->obstacle_mode != SMOKE_OBSTACLES_UNION)
         return fail(SMOKE_ERR_INVALID, "params: unknown obstacle_mode %d", p->obstacle_mode);
     if (p->iterations < 0 || p->iterations > 1000000)
@@ -205,6 +207,22 @@ smoke_status ensure_refine(smoke_sim* s)
     return SMOKE_OK;
 }
 
+// Allocates the vorticity-confinement scratch (once per simulation): four
+// FP32 padded fields, every cell of which the curl pass writes before it is
+// read, so nothing needs clearing.
+smoke_status ensure_vort(smoke_sim* s)
+{
+    if (s->vort_mem) return SMOKE_OK;
+    const size_t field = align256((size_t)s->g.elems * sizeof(float));
+    char* m = nullptr;
+    if (hipMalloc((void**)&m, 4 * field) != hipSuccess)
+        return fail(SMOKE_ERR_NOMEM, "vorticity: cannot allocate %.2f GiB", 4 * field / 1073741824.0);
+    s->vort_mem = m;
+    s->vb.mag = reinterpret_cast<float*>(m);
+    for (int k = 0; k < 3; ++k) s->vb.f[k] = reinterpret_cast<float*>(m + (k + 1) * field);
+    return SMOKE_OK;
+}
+
 }  // namespace
 
 namespace {
@@ -238,9 +256,13 @@ hipError_t stage_force(smoke_sim* s, float dt, long long* launches)
     a.gate = s->prm.density_gate;
     a.gate_on = s->prm.density_gate > 0.0f;
     const int n = s->now;
-    *launches += 1;
+    const float eps = s->prm.vorticity;
+    if (eps > 0.0f && !s->vort_mem) return hipErrorNotInitialized;  // set_params allocates it
+    *launches += eps > 0.0f ? 4 : 1;
     return smk::with_fields(s, [&](auto rho, auto u, auto v, auto w) {
-        return smk::force(s->g, u[n], v[n], w[n], rho[n], s->flags, a, s->stream);
+        hipError_t e = smk::force(s->g, u[n], v[n], w[n], rho[n], s->flags, a, s->stream);
+        if (e != hipSuccess || !(eps > 0.0f)) return e;
+        return smk::vorticity(s->g, u[n], v[n], w[n], s->flags, s->vb, eps, dt, s->stream);
     });
 }
 
@@ -540,7 +562,7 @@ smoke_status smoke_params_default(smoke_params* out)
     out->obstacle_mode = SMOKE_OBSTACLES_LAST_WINS;
     out->warm_start = 0;
     out->inner_sweeps = 20;
-    out->reserved = 0;
+    out->vorticity = 0.0f;
     out->tolerance = 0.0;
     return SMOKE_OK;
 }
@@ -664,6 +686,7 @@ void smoke_destroy(smoke_sim* s)
     if (s->rb_dev) (void)hipFree(s->rb_dev);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->refine_mem) (void)hipFree(s->refine_mem);
+    if (s->vort_mem) (void)hipFree(s->vort_mem);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -767,6 +790,10 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
         smoke_status rs = ensure_refine(s);
         if (rs != SMOKE_OK) return rs;
     }
+    if (p->vorticity > 0.0f) {
+        smoke_status vs = ensure_vort(s);
+        if (vs != SMOKE_OK) return vs;
+    }
     s->prm = *p;
     return SMOKE_OK;
 }
@@ -850,6 +877,7 @@ smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
     smoke_status st = check_params(&h.prm);
     if (st != SMOKE_OK) return st;
     if (h.prm.projection == SMOKE_PROJ_REFINED && (st = ensure_refine(s)) != SMOKE_OK) return st;
+    if (h.prm.vorticity > 0.0f && (st = ensure_vort(s)) != SMOKE_OK) return st;
     const char* p = static_cast<const char*>(host) + sizeof h;
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->objects.assign(reinterpret_cast<const Object*>(p), reinterpret_cast<const Object*>(p) + h.n_objects);

@@ -55,6 +55,10 @@ struct smoke_sim {
     // REFINED).
     char* refine_mem = nullptr;
     smk::RefineBufs rb;
+    // Vorticity-confinement scratch (allocated when vorticity is first set
+    // above 0).
+    char* vort_mem = nullptr;
+    smk::VortBufs vb{};
     float* staging = nullptr;
     uint8_t* base = nullptr;
     uint8_t* s = nullptr;

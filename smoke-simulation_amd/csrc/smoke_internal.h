@@ -175,6 +175,17 @@ bool source_box(const Grid& g, float4 sph, int lo[3], int ext[3]);
 hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
                  const uint8_t* flags, const ForceArgs& a, hipStream_t st);
 
+// Vorticity confinement on [now], after the force stage (an extension; three
+// passes over four FP32 padded scratch fields).
+struct VortBufs {
+    float* mag;   // |omega| per cell
+    float* f[3];  // omega, then the confinement force, per cell
+};
+hipError_t vorticity(const Grid& g, float* u, float* v, float* w, const uint8_t* flags,
+                     const VortBufs& b, float eps, float dt, hipStream_t st);
+hipError_t vorticity(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* flags,
+                     const VortBufs& b, float eps, float dt, hipStream_t st);
+
 // Projection.
 hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
                                   const uint8_t* flags, int colour, double omega, hipStream_t st);

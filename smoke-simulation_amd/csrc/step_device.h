@@ -669,5 +669,104 @@ __device__ __forceinline__ void advect_density_at(const Grid& g, const T* __rest
     advect_density_col<1>(g, r0, r1, u, v, w, flags, dt, x, y, z);
 }
 
+// Vorticity confinement, an extension: the reference lists it as future work
+// (README.md:33) and has no code for it, so the oracle's
+// vorticity_confinement (oracle/smoke_oracle.cpp) defines it and these
+// bodies follow its operations in its order.  With cell-centred velocities
+// c = mean of a cell's two faces, omega = curl c by central differences on
+// interior fluid cells (0 elsewhere), eta = grad |omega|, N = eta/|eta| where
+// |eta| > kVortMinGrad, F = eps (N x omega); every face between two fluid
+// cells gains dt * (mean of its two cells' F).  Three passes: curl (omega
+// kept in the F buffers, |omega| in mag), force (F over omega in place: a
+// cell reads only its own omega) and face update.
+constexpr float kVortMinGrad = 1e-6f;
+
+template <typename T>
+__device__ __forceinline__ float vort_cen(const T* __restrict__ a, int i, int step)
+{
+    return (ldf(a, i) + ldf(a, i + step)) * 0.5f;
+}
+// omega at interior cell i (every face read lies inside the padded field).
+template <typename T>
+__device__ __forceinline__ void vort_curl(const Grid& g, const T* __restrict__ u,
+                                          const T* __restrict__ v, const T* __restrict__ w, int i,
+                                          float& ox, float& oy, float& oz)
+{
+    const int X = 1, Y = g.px, Z = g.plane;
+    ox = (vort_cen(w, i + Y, Z) - vort_cen(w, i - Y, Z)) * 0.5f -
+         (vort_cen(v, i + Z, Y) - vort_cen(v, i - Z, Y)) * 0.5f;
+    oy = (vort_cen(u, i + Z, X) - vort_cen(u, i - Z, X)) * 0.5f -
+         (vort_cen(w, i + X, Z) - vort_cen(w, i - X, Z)) * 0.5f;
+    oz = (vort_cen(v, i + X, Y) - vort_cen(v, i - X, Y)) * 0.5f -
+         (vort_cen(u, i + Y, X) - vort_cen(u, i - Y, X)) * 0.5f;
+}
+// Pass 1 at cell (x,y,z) of [0,nx)x[0,ny)x[0,nz): writes mag and omega (zeros
+// off the interior fluid cells).
+template <typename T>
+__device__ __forceinline__ void vort_curl_at(const Grid& g, const T* __restrict__ u,
+                                             const T* __restrict__ v, const T* __restrict__ w,
+                                             const uint8_t* __restrict__ flags,
+                                             float* __restrict__ mag, float* __restrict__ f0,
+                                             float* __restrict__ f1, float* __restrict__ f2, int x,
+                                             int y, int z)
+{
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    const int i = g.idx(x, y, z);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, m = 0.0f;
+    if (is_interior(g, x, y, z) && (flags[i] & F_SELF)) {
+        vort_curl(g, u, v, w, i, ox, oy, oz);
+        m = sqrtf(ox * ox + oy * oy + oz * oz);
+    }
+    mag[i] = m;
+    f0[i] = ox;
+    f1[i] = oy;
+    f2[i] = oz;
+}
+// Pass 2 at interior fluid cells: F = eps (N x omega), or 0 where |eta| is at
+// rounding-noise level.
+__device__ __forceinline__ void vort_force_at(const Grid& g, const uint8_t* __restrict__ flags,
+                                              const float* __restrict__ mag,
+                                              float* __restrict__ f0, float* __restrict__ f1,
+                                              float* __restrict__ f2, float eps, int x, int y,
+                                              int z)
+{
+    if (!is_interior(g, x, y, z)) return;
+    const int i = g.idx(x, y, z);
+    if (!(flags[i] & F_SELF)) return;
+    const float ex = (mag[i + 1] - mag[i - 1]) * 0.5f;
+    const float ey = (mag[i + g.px] - mag[i - g.px]) * 0.5f;
+    const float ez = (mag[i + g.plane] - mag[i - g.plane]) * 0.5f;
+    const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+    if (!(len > kVortMinGrad)) {
+        f0[i] = 0.0f;
+        f1[i] = 0.0f;
+        f2[i] = 0.0f;
+        return;
+    }
+    const float nx = ex / len, ny = ey / len, nz = ez / len;
+    const float ox = f0[i], oy = f1[i], oz = f2[i];
+    f0[i] = eps * (ny * oz - nz * oy);
+    f1[i] = eps * (nz * ox - nx * oz);
+    f2[i] = eps * (nx * oy - ny * ox);
+}
+// Pass 3 at the low faces of cell (x,y,z): u/v/w += dt * mean F of the two
+// fluid cells the face separates.
+template <typename T>
+__device__ __forceinline__ void vort_apply_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
+                                              T* __restrict__ w, const uint8_t* __restrict__ flags,
+                                              const float* __restrict__ f0,
+                                              const float* __restrict__ f1,
+                                              const float* __restrict__ f2, float dt, int x, int y,
+                                              int z)
+{
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    const int i = g.idx(x, y, z);
+    const uint8_t fl = flags[i];
+    if (!(fl & F_SELF)) return;
+    if (fl & F_XM) stf(u, i, ldf(u, i) + dt * ((f0[i - 1] + f0[i]) * 0.5f));
+    if (fl & F_YM) stf(v, i, ldf(v, i) + dt * ((f1[i - g.px] + f1[i]) * 0.5f));
+    if (fl & F_ZM) stf(w, i, ldf(w, i) + dt * ((f2[i - g.plane] + f2[i]) * 0.5f));
+}
+
 }  // namespace dev
 }  // namespace smk

@@ -192,6 +192,32 @@ __global__ void k_force(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restr
                        blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ);
 }
 
+// Vorticity confinement (an extension with no reference code; the bodies in
+// step_device.h define it): curl, force and face passes over one simulation.
+template <typename T>
+__global__ void k_vort_curl(Grid g, const T* __restrict__ u, const T* __restrict__ v,
+                            const T* __restrict__ w, const uint8_t* __restrict__ flags,
+                            VortBufs b)
+{
+    dev::vort_curl_at(g, u, v, w, flags, b.mag, b.f[0], b.f[1], b.f[2],
+                      blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
+}
+
+__global__ void k_vort_force(Grid g, const uint8_t* __restrict__ flags, VortBufs b, float eps)
+{
+    dev::vort_force_at(g, flags, b.mag, b.f[0], b.f[1], b.f[2], eps,
+                       blockIdx.x * BX + threadIdx.x + 1, blockIdx.y * BY + threadIdx.y + 1,
+                       blockIdx.z + 1);
+}
+
+template <typename T>
+__global__ void k_vort_apply(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                             const uint8_t* __restrict__ flags, VortBufs b, float dt)
+{
+    dev::vort_apply_at(g, u, v, w, flags, b.f[0], b.f[1], b.f[2], dt,
+                       blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, blockIdx.z);
+}
+
 // divergence (smokeSimulation.cu:356-394): one red-black SOR half-sweep in
 // place on the face velocities.  Same-colour cells share no face, so the
 // in-place update is race-free.
@@ -504,6 +530,33 @@ hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const bf16s* rho,
                  const uint8_t* flags, const ForceArgs& a, hipStream_t st)
 {
     return launch_force(g, u, v, w, rho, flags, a, st);
+}
+
+template <typename T>
+static hipError_t launch_vorticity(const Grid& g, T* u, T* v, T* w, const uint8_t* flags,
+                                   const VortBufs& b, float eps, float dt, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vort_curl<T>, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
+                       flags, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vort_force, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
+                       flags, b, eps);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vort_apply<T>, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
+                       flags, b, dt);
+    return hipGetLastError();
+}
+hipError_t vorticity(const Grid& g, float* u, float* v, float* w, const uint8_t* flags,
+                     const VortBufs& b, float eps, float dt, hipStream_t st)
+{
+    return launch_vorticity(g, u, v, w, flags, b, eps, dt, st);
+}
+hipError_t vorticity(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* flags,
+                     const VortBufs& b, float eps, float dt, hipStream_t st)
+{
+    return launch_vorticity(g, u, v, w, flags, b, eps, dt, st);
 }
 
 hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,

@@ -51,7 +51,7 @@ class Params(ctypes.Structure):
         ("obstacle_mode", ctypes.c_int),
         ("warm_start", ctypes.c_int),
         ("inner_sweeps", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("vorticity", ctypes.c_float),
         ("tolerance", ctypes.c_double),
     ]
 

@@ -198,8 +198,23 @@ def case_precision():
     g.close()
 
 
+def case_vorticity():
+    # Vorticity confinement after the force stage, under each projection.
+    for proj, its, dims, seed in [(0, 4, (14, 12, 11), 30), (1, 4, (11, 15, 9), 31),
+                                  (2, 6, (16, 10, 12), 32)]:
+        sc = scenes.test_scene(dims, seed=seed, projection=proj, iterations=its, noise=0.5)
+        sc["params"]["vorticity"] = 1.5
+        g, o = gpu_for(sc), oracle_for(sc, 32)
+        for _ in range(2):
+            g.step(sc["dt"], readback=False)
+        o.step(sc["dt"], 2)
+        assert_bitwise_equal(g.state(), o.state(), f"vorticity p{proj}")
+        g.close()
+
+
 CASES = {"single": case_single, "plans": case_plans, "batch": case_batch,
-         "server": case_server, "readback": case_readback, "precision": case_precision}
+         "server": case_server, "readback": case_readback, "precision": case_precision,
+         "vorticity": case_vorticity}
 
 
 def main():

@@ -432,3 +432,67 @@ def test_fp32_tracks_fp64(proj):
     for k in range(3):
         va, vb = a.get_velocity(k, 1 - now), b.get_velocity(k, 1 - now)
         assert np.linalg.norm(va - vb) / np.linalg.norm(va) < 1e-3
+
+
+# ------------------------------------------------------ vorticity confinement
+# The oracle's vorticity confinement (an addition: the reference lists it as
+# future work, README.md:33) is pinned by closed forms: a velocity whose
+# vorticity grows linearly in x gets the force eps * (N x omega) with
+# N = +x exactly, and a solid-body rotation (uniform vorticity) gets none.
+
+def _force_stage(o, dt):
+    o.stage(orc.STAGE_SCENE, dt)
+    o.stage(orc.STAGE_FORCE, dt)
+
+
+def test_vorticity_confinement_linear_vorticity_closed_form():
+    nx, ny, nz = 14, 11, 10
+    eps, c, dt = 0.5, 0.01, 0.25
+    o = fresh((nx, ny, nz), 64, vorticity=eps, gravity=0.0, buoyancy=0.0, max_velocity=1e6)
+    xs = np.arange(nx + 1, dtype=np.float64)
+    V = np.broadcast_to(c * (xs + 0.5) ** 2, (nz + 1, ny + 1, nx + 1)).copy()
+    o.set_velocity(1, 0, V)
+    _force_stage(o, dt)
+    got = o.get_velocity(1, 0)
+    # omega_z(x) = 2c(x + 1/2) on interior cells, grad|omega| = (2c, 0, 0) away
+    # from the walls, so F_y = -eps * omega_z and each interior v-face moves by
+    # dt * F_y of its (identical) two cells.
+    for x in range(2, nx - 2):
+        want = V[3:nz - 3, 3:ny - 3, x] - dt * eps * 2 * c * (x + 0.5)
+        np.testing.assert_allclose(got[3:nz - 3, 3:ny - 3, x], want, rtol=0, atol=1e-14)
+    for k in (0, 2):  # F_x = F_z = 0 in the interior: u and w untouched there
+        assert np.array_equal(o.get_velocity(k, 0)[3:nz - 3, 3:ny - 3, 3:nx - 3], 0 * V[3:nz - 3, 3:ny - 3, 3:nx - 3])
+
+
+def test_vorticity_confinement_leaves_uniform_vorticity_alone():
+    nx, ny, nz = 13, 12, 9
+    om = 0.02
+    o = fresh((nx, ny, nz), 64, vorticity=0.7, gravity=0.0, buoyancy=0.0, max_velocity=1e6)
+    yc, xc = ny / 2, nx / 2
+    U = np.zeros((nz + 1, ny + 1, nx + 1))
+    V = np.zeros_like(U)
+    U[:, :, :] = (-om * (np.arange(ny + 1) + 0.5 - yc))[None, :, None]   # u at (x, y+.5, z+.5)
+    V[:, :, :] = (om * (np.arange(nx + 1) + 0.5 - xc))[None, None, :]    # v at (x+.5, y, z+.5)
+    o.set_velocity(0, 0, U)
+    o.set_velocity(1, 0, V)
+    _force_stage(o, 0.5)
+    # omega = (0, 0, 2*om) everywhere inside: grad|omega| = 0 away from the
+    # walls, N = 0 there, so interior faces keep their values.
+    sl = (slice(3, nz - 3), slice(3, ny - 3), slice(3, nx - 3))
+    assert np.array_equal(o.get_velocity(0, 0)[sl], U[sl])
+    assert np.array_equal(o.get_velocity(1, 0)[sl], V[sl])
+
+
+def test_vorticity_off_is_the_reference_force_stage():
+    dims = (11, 10, 9)
+    a = fresh(dims, 32, gravity=7.8, buoyancy=2.0)
+    b = fresh(dims, 32, gravity=7.8, buoyancy=2.0, vorticity=0.0)
+    rng = np.random.default_rng(5)
+    for o in (a, b):
+        for k in range(3):
+            o.set_velocity(k, 0, np.random.default_rng(k).uniform(-0.1, 0.1, (10, 11, 12)))
+        o.set_density(0, rng.uniform(0, 1, (9, 10, 11)))
+        rng = np.random.default_rng(5)
+        _force_stage(o, 0.05)
+    for k in range(3):
+        assert np.array_equal(a.get_velocity(k, 0), b.get_velocity(k, 0))
