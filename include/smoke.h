@@ -90,6 +90,13 @@ typedef struct smoke_params {
                            the reference (README.md:33 lists it as future
                            work)                                              */
     double tolerance;   /* REFINED: stop at this relative residual; 0: run all */
+    int obstacle_velocity; /* 1: moving-obstacle velocity boundary condition:
+                           each step, after the mask is built, every face of
+                           [now] next to a cell an obstacle made solid takes
+                           that obstacle's velocity (the high-side cell's
+                           obstacle when both are).  0 = off, the reference
+                           (it stores obstacle velocities and never reads
+                           them, smokeSimulation.cu:45-52, :299-301)         */
 } smoke_params;
 
 /* Step stages, in execution order (for stage-level testing and timing). */
@@ -199,6 +206,11 @@ smoke_status smoke_add_source(smoke_sim* sim, float x, float y, float z, float r
 /* Replaces updateObjectPos (smokeSimulation.cu:106-109), with a bounds
  * check. */
 smoke_status smoke_move_object(smoke_sim* sim, int id, float x, float y, float z);
+
+/* Sets the velocity an object was added with (addObstacle's vx, vy, vz,
+ * smokeSimulation.cu:88-97); read by the moving-obstacle boundary condition.
+ * No reference counterpart: the reference cannot change it after adding. */
+smoke_status smoke_set_object_velocity(smoke_sim* sim, int id, float vx, float vy, float vz);
 
 /* Base solid mask, nx*ny*nz bytes, nonzero = solid.  Obstacles are combined
  * with it each step (reference: the mask is the y = 0 floor only,

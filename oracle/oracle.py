@@ -34,13 +34,14 @@ class OrcParams(ctypes.Structure):
         ("obstacle_mode", ctypes.c_int),
         ("warm_start", ctypes.c_int),
         ("vorticity", ctypes.c_float),
+        ("obstacle_velocity", ctypes.c_int),
     ]
 
 
 DEFAULT_PARAMS = dict(
     gravity=-9.82, buoyancy=2.0, density_gate=0.0, max_velocity=3.0,
     sor_omega=1.9, jacobi_omega=1.0, projection=PROJ_REFERENCE, iterations=30,
-    obstacle_mode=OBST_LAST_WINS, warm_start=0, vorticity=0.0,
+    obstacle_mode=OBST_LAST_WINS, warm_start=0, vorticity=0.0, obstacle_velocity=0,
 )
 
 _lib = None
@@ -64,6 +65,7 @@ def load():
         "orc_set_params": (None, [vp, ctypes.POINTER(OrcParams)]),
         "orc_add_object": (i, [vp, i, f, f, f, f, f, f, f]),
         "orc_move_object": (i, [vp, i, f, f, f]),
+        "orc_set_object_velocity": (i, [vp, i, f, f, f]),
         "orc_set_solids": (None, [vp, u8p]),
         "orc_rebuild_mask": (None, [vp]),
         "orc_set_density": (None, [vp, i, dp]),
@@ -152,6 +154,10 @@ class Oracle:
 
     def move_object(self, oid, x, y, z):
         if self.lib.orc_move_object(self.h, oid, x, y, z) != 0:
+            raise IndexError(oid)
+
+    def set_object_velocity(self, oid, vx, vy, vz):
+        if self.lib.orc_set_object_velocity(self.h, oid, vx, vy, vz) != 0:
             raise IndexError(oid)
 
     def set_solids(self, solid):

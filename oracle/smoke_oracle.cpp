@@ -97,6 +97,7 @@ struct Params {
     int obstacle_mode;    // ObstacleMode
     int warm_start;       // pressure forms: start from the previous P
     float vorticity;      // vorticity-confinement strength eps; 0 = off (reference)
+    int obstacle_velocity;  // 1: moving-obstacle velocity boundary condition; 0 = off (reference)
 };
 
 struct Sphere {
@@ -121,6 +122,7 @@ Params default_params()
     p.obstacle_mode = OBST_LAST_WINS;
     p.warm_start = 0;
     p.vorticity = 0.0f;
+    p.obstacle_velocity = 0;
     return p;
 }
 
@@ -130,6 +132,8 @@ struct SimBase {
     long Sx = 0, Sy = 0, Sz = 0;
     long N = 0, NS = 0;
     std::vector<uint8_t> base, s, flags;
+    // Per cell: id of the obstacle whose sphere made the cell solid (-1: none).
+    std::vector<int> owner;
     std::vector<Sphere> objects;
     int now = 1;  // indexNow, smokeSimulation.cu:707
     Params prm = default_params();
@@ -174,17 +178,24 @@ struct SimBase {
                 for (int x = 0; x < nx; ++x) {
                     long i = c(x, y, z);
                     uint8_t v = base[i];
+                    int who = -1;
                     if (!obst.empty() && interior(x, y, z)) {
                         bool solid;
                         if (prm.obstacle_mode == OBST_LAST_WINS) {
                             solid = inside(*obst.back(), x, y, z);
+                            if (solid) who = obst.back()->id;
                         } else {
                             solid = false;
-                            for (const Sphere* o : obst) solid = solid || inside(*o, x, y, z);
+                            for (const Sphere* o : obst)
+                                if (inside(*o, x, y, z)) {
+                                    solid = true;
+                                    who = o->id;  // the last obstacle containing the cell
+                                }
                         }
                         if (solid) v = 0;
                     }
                     s[i] = v;
+                    owner[i] = who;
                 }
         // Flags: neighbour bits exist only where the neighbour exists.
 #pragma omp parallel for schedule(static)
@@ -234,6 +245,7 @@ struct Sim : SimBase {
             for (int x = 0; x < nx; ++x) base[c(x, 0, z)] = 0;
         s = base;
         flags.assign(N, 0);
+        owner.assign(N, -1);
         rebuild_mask();
     }
 
@@ -291,6 +303,33 @@ struct Sim : SimBase {
                             }
         }
         rebuild_mask();
+        if (prm.obstacle_velocity) obstacle_velocity_bc();
+    }
+
+    // Moving-obstacle velocity boundary condition, an extension: the
+    // reference stores each obstacle's velocity (smokeSimulation.cu:45-52)
+    // but never uses it (:299-301).  With it on, after the mask is built, every
+    // face of [now] between two cells at least one of which an obstacle made
+    // solid takes that obstacle's velocity component (the high-side cell's
+    // obstacle when both are).  The projection leaves faces next to solid
+    // cells alone and the advection writes only fluid-fluid faces, so the
+    // obstacle's motion enters the divergence the projection removes.
+    void obstacle_velocity_bc()
+    {
+        const long step[3] = {1, nx, (long)nx * ny};
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const long i = c(x, y, z);
+                    const int pos[3] = {x, y, z};
+                    for (int k = 0; k < 3; ++k) {
+                        if (pos[k] < 1) continue;  // the low cell is outside the grid
+                        const int who = owner[i] >= 0 ? owner[i] : owner[i - step[k]];
+                        if (who < 0) continue;
+                        comp(k, now)[f(x, y, z)] = (Real)objects[who].vel[k];
+                    }
+                }
     }
 
     // integrate (smokeSimulation.cu:315-329) then velocityConfinement
@@ -718,6 +757,7 @@ struct orc_params {
     double sor_omega, jacobi_omega;
     int projection, iterations, obstacle_mode, warm_start;
     float vorticity;
+    int obstacle_velocity;
 };
 
 void* orc_create(int real_bits, int nx, int ny, int nz)
@@ -762,6 +802,7 @@ void orc_set_params(void* h, const orc_params* p)
     s->prm.obstacle_mode = p->obstacle_mode;
     s->prm.warm_start = p->warm_start;
     s->prm.vorticity = p->vorticity;
+    s->prm.obstacle_velocity = p->obstacle_velocity;
 }
 
 // addObstacle / addSmokeSource (smokeSimulation.cu:88-104): ids are shared
@@ -787,6 +828,18 @@ int orc_move_object(void* h, int id, float x, float y, float z)
     s->objects[id].pos[0] = x;
     s->objects[id].pos[1] = y;
     s->objects[id].pos[2] = z;
+    return 0;
+}
+
+// Sets the stored velocity of an object (used by the moving-obstacle
+// boundary condition).
+int orc_set_object_velocity(void* h, int id, float vx, float vy, float vz)
+{
+    SimBase* s = static_cast<SimBase*>(h);
+    if (id < 0 || id >= (int)s->objects.size()) return -1;
+    s->objects[id].vel[0] = vx;
+    s->objects[id].vel[1] = vy;
+    s->objects[id].vel[2] = vz;
     return 0;
 }
 

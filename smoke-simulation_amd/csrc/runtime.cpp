@@ -108,6 +108,8 @@ smoke_status check_params(const smoke_params* p)
         if (p->iterations < 1)
             return fail(SMOKE_ERR_INVALID, "params: REFINED needs iterations >= 1");
     }
+    if (p->obstacle_velocity != 0 && p->obstacle_velocity != 1)
+        return fail(SMOKE_ERR_INVALID, "params: obstacle_velocity must be 0 or 1");
     if (!std::isfinite(p->vorticity) || p->vorticity < 0.0f)
         return fail(SMOKE_ERR_INVALID, "params: vorticity must be finite and >= 0");
     if (p->obstacle_mode != SMOKE_OBSTACLES_LAST_WINS && p->obstacle_mode != SMOKE_OBSTACLES_UNION)
@@ -165,6 +167,11 @@ hipError_t smk::refresh_mask(smoke_sim* s, long long* launches)
         if (e != hipSuccess) return e;
         if (launches) *launches += 1;
     }
+    if (s->owner) {
+        e = smk::build_owner(s->g, s->owner, obst, s->prm.obstacle_mode, s->stream);
+        if (e != hipSuccess) return e;
+        if (launches) *launches += 1;
+    }
     s->mask_dirty = false;
     return hipSuccess;
 }
@@ -207,6 +214,19 @@ smoke_status ensure_refine(smoke_sim* s)
     return SMOKE_OK;
 }
 
+// Allocates the owner bytes of the moving-obstacle boundary condition (once
+// per simulation); the next mask refresh builds them.
+smoke_status ensure_owner(smoke_sim* s)
+{
+    if (s->owner) return SMOKE_OK;
+    if (hipMalloc((void**)&s->owner, (size_t)s->g.elems) != hipSuccess) {
+        s->owner = nullptr;
+        return fail(SMOKE_ERR_NOMEM, "obstacle_velocity: cannot allocate %d bytes", s->g.elems);
+    }
+    s->mask_dirty = true;
+    return SMOKE_OK;
+}
+
 // Allocates the vorticity-confinement scratch (once per simulation): four
 // FP32 padded fields, every cell of which the curl pass writes before it is
 // read, so nothing needs clearing.
@@ -243,7 +263,18 @@ hipError_t stage_scene(smoke_sim* s, long long* launches)
         if (e != hipSuccess) return e;
         *launches += launched;
     }
-    return refresh_mask(s, launches);
+    hipError_t e = refresh_mask(s, launches);
+    if (e != hipSuccess || !s->prm.obstacle_velocity) return e;
+    if (!s->owner) return hipErrorNotInitialized;  // set_params allocates it
+    smk::SphereList vel;
+    vel.count = 0;
+    for (const Object& o : s->objects)
+        if (o.type == 0) vel.s[vel.count++] = make_float4(o.vel[0], o.vel[1], o.vel[2], 0.0f);
+    *launches += 1;
+    const int n = s->now;
+    return smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+        return smk::obstacle_velocity_bc(s->g, u[n], v[n], w[n], s->owner, vel, s->stream);
+    });
 }
 
 hipError_t stage_force(smoke_sim* s, float dt, long long* launches)
@@ -563,6 +594,7 @@ smoke_status smoke_params_default(smoke_params* out)
     out->warm_start = 0;
     out->inner_sweeps = 20;
     out->vorticity = 0.0f;
+    out->obstacle_velocity = 0;
     out->tolerance = 0.0;
     return SMOKE_OK;
 }
@@ -687,6 +719,7 @@ void smoke_destroy(smoke_sim* s)
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->refine_mem) (void)hipFree(s->refine_mem);
     if (s->vort_mem) (void)hipFree(s->vort_mem);
+    if (s->owner) (void)hipFree(s->owner);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -794,6 +827,10 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
         smoke_status vs = ensure_vort(s);
         if (vs != SMOKE_OK) return vs;
     }
+    if (p->obstacle_velocity) {
+        smoke_status os = ensure_owner(s);
+        if (os != SMOKE_OK) return os;
+    }
     s->prm = *p;
     return SMOKE_OK;
 }
@@ -878,6 +915,7 @@ smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
     if (st != SMOKE_OK) return st;
     if (h.prm.projection == SMOKE_PROJ_REFINED && (st = ensure_refine(s)) != SMOKE_OK) return st;
     if (h.prm.vorticity > 0.0f && (st = ensure_vort(s)) != SMOKE_OK) return st;
+    if (h.prm.obstacle_velocity && (st = ensure_owner(s)) != SMOKE_OK) return st;
     const char* p = static_cast<const char*>(host) + sizeof h;
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->objects.assign(reinterpret_cast<const Object*>(p), reinterpret_cast<const Object*>(p) + h.n_objects);
@@ -1042,6 +1080,20 @@ smoke_status smoke_move_object(smoke_sim* s, int id, float x, float y, float z)
     Object& o = s->objects[id];
     o.pos[0] = x; o.pos[1] = y; o.pos[2] = z;
     if (o.type == 0) s->mask_dirty = true;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_set_object_velocity(smoke_sim* s, int id, float vx, float vy, float vz)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "set_object_velocity: NULL sim");
+    if (!std::isfinite(vx) || !std::isfinite(vy) || !std::isfinite(vz))
+        return fail(SMOKE_ERR_INVALID, "set_object_velocity: velocity must be finite");
+    std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_REFUSE_IN_BATCH(s, "set_object_velocity");
+    if (id < 0 || id >= (int)s->objects.size())
+        return fail(SMOKE_ERR_INVALID, "set_object_velocity: no object with id %d", id);
+    Object& o = s->objects[id];
+    o.vel[0] = vx; o.vel[1] = vy; o.vel[2] = vz;
     return SMOKE_OK;
 }
 

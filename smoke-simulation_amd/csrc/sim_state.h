@@ -59,6 +59,9 @@ struct smoke_sim {
     // above 0).
     char* vort_mem = nullptr;
     smk::VortBufs vb{};
+    // Moving-obstacle boundary condition: per-cell owner bytes (allocated
+    // when obstacle_velocity is first set; built with the mask).
+    uint8_t* owner = nullptr;
     float* staging = nullptr;
     uint8_t* base = nullptr;
     uint8_t* s = nullptr;

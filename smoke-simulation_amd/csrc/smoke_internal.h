@@ -175,6 +175,21 @@ bool source_box(const Grid& g, float4 sph, int lo[3], int ext[3]);
 hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
                  const uint8_t* flags, const ForceArgs& a, hipStream_t st);
 
+// Moving-obstacle velocity boundary condition (an extension; the reference
+// stores obstacle velocities and never reads them).  build_owner records per
+// cell the index in the obstacle list of the obstacle that made it solid
+// (kNoOwner: none), under the same rule as build_mask; obstacle_velocity_bc
+// then sets every face of [now] next to an owned cell to the owner's velocity
+// (the high-side cell's owner first).  `vel` holds (vx, vy, vz, 0) per
+// obstacle, in obstacle-list order.
+constexpr uint8_t kNoOwner = 255;
+hipError_t build_owner(const Grid& g, uint8_t* owner, const SphereList& obst, int mode,
+                       hipStream_t st);
+hipError_t obstacle_velocity_bc(const Grid& g, float* u, float* v, float* w, const uint8_t* owner,
+                                const SphereList& vel, hipStream_t st);
+hipError_t obstacle_velocity_bc(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* owner,
+                                const SphereList& vel, hipStream_t st);
+
 // Vorticity confinement on [now], after the force stage (an extension; three
 // passes over four FP32 padded scratch fields).
 struct VortBufs {

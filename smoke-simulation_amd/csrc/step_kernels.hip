@@ -169,6 +169,53 @@ __global__ void k_flags(Grid g, const uint8_t* __restrict__ s, uint8_t* __restri
     flags[i] = fl;
 }
 
+// Owner of each cell for the moving-obstacle boundary condition: the index
+// of the obstacle whose sphere makes it solid under k_mask's rule (LAST_WINS:
+// the last obstacle; UNION: the last one containing the cell), or kNoOwner.
+__global__ void k_owner(Grid g, uint8_t* __restrict__ owner, SphereList obst, int mode)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    uint8_t who = kNoOwner;
+    if (obst.count > 0 && dev::is_interior(g, x, y, z)) {
+        if (mode == SMOKE_OBSTACLES_LAST_WINS) {
+            if (dev::inside(obst.s[obst.count - 1], x, y, z)) who = (uint8_t)(obst.count - 1);
+        } else {
+            for (int k = 0; k < obst.count; ++k)
+                if (dev::inside(obst.s[k], x, y, z)) who = (uint8_t)k;
+        }
+    }
+    owner[g.idx(x, y, z)] = who;
+}
+
+// The low faces of cell (x,y,z) next to an owned cell take the owner's
+// velocity component (this cell's owner first, else the low neighbour's).
+template <typename T>
+__global__ void k_obstacle_bc(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                              const uint8_t* __restrict__ owner, SphereList vel)
+{
+    int x = blockIdx.x * BX + threadIdx.x;
+    int y = blockIdx.y * BY + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    const int i = g.idx(x, y, z);
+    const uint8_t self = owner[i];
+    if (x >= 1) {
+        const uint8_t o = self != kNoOwner ? self : owner[i - 1];
+        if (o != kNoOwner) dev::stf(u, i, vel.s[o].x);
+    }
+    if (y >= 1) {
+        const uint8_t o = self != kNoOwner ? self : owner[i - g.px];
+        if (o != kNoOwner) dev::stf(v, i, vel.s[o].y);
+    }
+    if (z >= 1) {
+        const uint8_t o = self != kNoOwner ? self : owner[i - g.plane];
+        if (o != kNoOwner) dev::stf(w, i, vel.s[o].z);
+    }
+}
+
 // fillSmoke (smokeSimulation.cu:251-273) restricted to the bounding box of
 // one source (clipped to the interior by the launcher).
 template <typename T>
@@ -462,6 +509,33 @@ hipError_t build_mask(const Grid& g, const uint8_t* base, uint8_t* s, const Sphe
     hipLaunchKernelGGL(k_mask, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, base, s, obst,
                        mode);
     return hipGetLastError();
+}
+
+hipError_t build_owner(const Grid& g, uint8_t* owner, const SphereList& obst, int mode,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_owner, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, owner, obst,
+                       mode);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_obstacle_bc(const Grid& g, T* u, T* v, T* w, const uint8_t* owner,
+                                     const SphereList& vel, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_obstacle_bc<T>, grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g, u, v,
+                       w, owner, vel);
+    return hipGetLastError();
+}
+hipError_t obstacle_velocity_bc(const Grid& g, float* u, float* v, float* w, const uint8_t* owner,
+                                const SphereList& vel, hipStream_t st)
+{
+    return launch_obstacle_bc(g, u, v, w, owner, vel, st);
+}
+hipError_t obstacle_velocity_bc(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* owner,
+                                const SphereList& vel, hipStream_t st)
+{
+    return launch_obstacle_bc(g, u, v, w, owner, vel, st);
 }
 
 hipError_t build_flags(const Grid& g, const uint8_t* s, uint8_t* flags, hipStream_t st)

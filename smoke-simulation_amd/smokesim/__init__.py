@@ -53,6 +53,7 @@ class Params(ctypes.Structure):
         ("inner_sweeps", ctypes.c_int),
         ("vorticity", ctypes.c_float),
         ("tolerance", ctypes.c_double),
+        ("obstacle_velocity", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -106,6 +107,7 @@ def _declare(lib):
         "smoke_add_obstacle": (st, [vp, f, f, f, f, f, f, f, ip]),
         "smoke_add_source": (st, [vp, f, f, f, f, ip]),
         "smoke_move_object": (st, [vp, i, f, f, f]),
+        "smoke_set_object_velocity": (st, [vp, i, f, f, f]),
         "smoke_set_solids": (st, [vp, u8p]),
         "smoke_set_density": (st, [vp, i, fp]),
         "smoke_get_density": (st, [vp, i, fp]),
@@ -317,6 +319,9 @@ class Simulation:
 
     def move_object(self, oid, x, y, z):
         _check(self.lib.smoke_move_object(self.h, int(oid), x, y, z))
+
+    def set_object_velocity(self, oid, vx, vy, vz):
+        _check(self.lib.smoke_set_object_velocity(self.h, int(oid), vx, vy, vz))
 
     def set_solids(self, solid):
         a = np.ascontiguousarray(solid, dtype=np.uint8).reshape(self.cell_shape)

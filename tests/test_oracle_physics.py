@@ -496,3 +496,89 @@ def test_vorticity_off_is_the_reference_force_stage():
         _force_stage(o, 0.05)
     for k in range(3):
         assert np.array_equal(a.get_velocity(k, 0), b.get_velocity(k, 0))
+
+
+# Moving-obstacle velocity boundary condition (an extension: the reference
+# stores obstacle velocities, smokeSimulation.cu:45-52, and never reads them,
+# :299-301).  Pinned by a brute-force restatement of which faces an obstacle
+# owns (sphere membership evaluated with numpy float32 scalars) and by the
+# flow it induces after a converged projection: divergence-free fluid cells
+# and the dipole pattern of a sphere pushed through still fluid.
+
+def _sphere_cells(dims, sph):
+    nx, ny, nz = dims
+    cx, cy, cz, r = (np.float32(t) for t in sph)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    dx = x.astype(np.float32) - cx
+    dy = y.astype(np.float32) - cy
+    dz = z.astype(np.float32) - cz
+    dist = (dx * dx + dy * dy) + dz * dz
+    inner = np.zeros(dims[::-1], bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    return (dist < r * r) & inner
+
+
+@pytest.mark.parametrize("mode", [orc.OBST_LAST_WINS, orc.OBST_UNION])
+def test_obstacle_velocity_bc_owns_exactly_the_obstacle_faces(mode):
+    dims = (13, 11, 10)
+    nx, ny, nz = dims
+    obst = [((4.0, 5.0, 5.0, 3.0), (0.25, -0.5, 0.75)), ((7.5, 4.5, 4.0, 2.6), (-1.0, 0.5, 2.0))]
+    o = fresh(dims, 64, obstacle_velocity=1, obstacle_mode=mode)
+    ids = [o.add_obstacle(*sph[:3], sph[3], vel=vel) for sph, vel in obst]
+    init = [np.random.default_rng(10 + k).uniform(-1, 1, (nz + 1, ny + 1, nx + 1)) for k in range(3)]
+    for k in range(3):
+        o.set_velocity(k, 0, init[k])
+    o.stage(orc.STAGE_SCENE, 0.05)  # swaps to half 0 and applies the condition there
+    now = o.index_now
+    assert now == 0
+    owner = -np.ones(dims[::-1], int)
+    used = obst[-1:] if mode == orc.OBST_LAST_WINS else obst
+    base = len(obst) - len(used)
+    for j, (sph, _) in enumerate(used):
+        owner[_sphere_cells(dims, sph)] = ids[base + j]
+    vel = {ids[j]: obst[j][1] for j in range(len(obst))}
+    for k in range(3):
+        want = init[k].copy()
+        for z in range(nz):
+            for y in range(ny):
+                for x in range(nx):
+                    p = (x, y, z)
+                    if p[k] < 1:
+                        continue
+                    lo = list(p)
+                    lo[k] -= 1
+                    who = owner[z, y, x] if owner[z, y, x] >= 0 else owner[lo[2], lo[1], lo[0]]
+                    if who >= 0:
+                        want[z, y, x] = np.float32(vel[who][k])
+        assert np.array_equal(o.get_velocity(k, now), want), k
+    assert np.array_equal(o.get_mask() == 0, (owner >= 0) | (o.get_mask() == 0))
+
+
+def test_obstacle_velocity_bc_off_leaves_faces_alone():
+    dims = (10, 9, 8)
+    o = fresh(dims, 64)
+    o.add_obstacle(5.0, 4.0, 4.0, 2.5, vel=(1.0, 2.0, 3.0))
+    init = np.random.default_rng(3).uniform(-1, 1, (9, 10, 11))
+    o.set_velocity(0, 0, init)
+    o.stage(orc.STAGE_SCENE, 0.05)
+    o.stage(orc.STAGE_SCENE, 0.05)
+    assert np.array_equal(o.get_velocity(0, 0), init)
+
+
+def test_obstacle_velocity_bc_pushes_still_fluid_divergence_free():
+    n = 16
+    o = fresh((n, n, n), 64, projection=orc.PROJ_REFERENCE, iterations=1500, obstacle_velocity=1,
+              gravity=0.0, buoyancy=0.0)
+    o.add_obstacle(8.0, 8.0, 8.0, 3.2, vel=(1.0, 0.0, 0.0))
+    for st in (orc.STAGE_SCENE, orc.STAGE_FORCE, orc.STAGE_PROJECT):
+        o.stage(st, 0.05)
+    now = o.index_now
+    u, v, w = (o.get_velocity(k, now) for k in range(3))
+    act = (o.get_flags() & orc.F_ACTIVE) != 0
+    assert np.abs(cell_div(u, v, w)[act]).max() < 1e-10
+    # A sphere moving +x through still fluid: fluid ahead and behind moves
+    # with it, fluid beside it flows back.
+    # Obstacle cells on the centre row are x = 5..11.
+    assert u[8, 8, 12] == 1.0 and u[8, 8, 5] == 1.0           # obstacle faces
+    assert u[8, 8, 13] > 0.2 and u[8, 8, 4] > 0.2             # ahead, behind
+    assert u[8, 12, 8] < -0.05 and u[12, 8, 8] < -0.05 and u[4, 8, 8] < -0.05  # beside
