@@ -389,6 +389,45 @@ smoke_status smoke_server_stats_get(smoke_server* server, smoke_server_stats* ou
 smoke_status smoke_server_tick_ms(smoke_server* server, double* ms, int* requests, int cap,
                                   int* count);
 
+/* ------------------------------------------ proxy-plane slicing geometry */
+/* The renderer's view-aligned slices of the volume's bounding box,
+ * BoundingBox::generateProxyGeometry (project/boundingBox.cpp:133-183), on
+ * the GPU: `steps` planes perpendicular to the view direction, from the
+ * nearest to the farthest corner of a box of `box_dimensions` centred at the
+ * origin (getMinMaxPoints, :231-241), each cut with the box at
+ * `box_position` (planeBoxIntersections, :285-354), its corners ordered by
+ * pseudoangle about their mean (sortPoints, :259-272) and fanned into
+ * triangles (teselate, :243-255).  Output as the reference's three arrays:
+ * vertices (3 floats each; per plane its corners, then their mean), 16-bit
+ * triangle indices into the whole vertex array, and one smoke_proxy_plane
+ * per plane (m_planeIndexing).  Bit-identical to the sequential CPU
+ * computation in FP32.  A plane that misses the box yields its (NaN) mean
+ * and no triangles (the reference's teselate is undefined there). */
+typedef struct smoke_proxy_plane {
+    int num_triangles;    /* corners of the cut (PlaneIndexValue, boundingBox.h:22-25) */
+    int triangle_offset;  /* first triangle of the plane in the index array   */
+} smoke_proxy_plane;
+
+#define SMOKE_PROXY_MAX_STEPS 4096
+#define SMOKE_PROXY_VERTICES_PER_PLANE 13  /* capacity per plane, worst case */
+#define SMOKE_PROXY_INDICES_PER_PLANE 36
+
+/* Host arrays: vertices >= 3*13*steps floats, indices >= 36*steps,
+ * planes >= steps; *n_vertices and *n_indices receive the counts used.
+ * Replaces the body of generateProxyGeometry; synchronous. */
+smoke_status smoke_proxy_geometry(const float box_position[3], const float box_dimensions[3],
+                                  const float view_origin[3], const float view_direction[3],
+                                  int steps, float* vertices, int16_t* indices,
+                                  smoke_proxy_plane* planes, int* n_vertices, int* n_indices);
+/* Device arrays of the same capacities (e.g. buffers mapped from the
+ * renderer), and `counts` = 2 device ints (vertices, indices).  Enqueued on
+ * `stream` (a hipStream_t; NULL = the default stream); returns without
+ * waiting.  Allocates nothing. */
+smoke_status smoke_proxy_geometry_device(const float box_position[3], const float box_dimensions[3],
+                                         const float view_origin[3], const float view_direction[3],
+                                         int steps, float* vertices, int16_t* indices,
+                                         smoke_proxy_plane* planes, int* counts, void* stream);
+
 /* ------------------------------------------------------ legacy interface */
 /* Process-global single simulation behind the reference's nine functions
  * (smokeSimulation.cuh:4-17).  Used by smokeSimulation.cuh; errors are

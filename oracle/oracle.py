@@ -55,6 +55,7 @@ def load():
         raise RuntimeError(f"oracle library not built: {_LIB_PATH} (run make -C oracle)")
     lib = ctypes.CDLL(_LIB_PATH)
     vp, i, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    fp = ctypes.POINTER(ctypes.c_float)
     dp = ctypes.POINTER(ctypes.c_double)
     u8p = ctypes.POINTER(ctypes.c_uint8)
     sig = {
@@ -80,6 +81,10 @@ def load():
         "orc_stage": (None, [vp, i, f]),
         "orc_step": (None, [vp, f]),
         "orc_steps": (None, [vp, f, i]),
+        "orc_plane_box": (i, [fp, fp, fp, fp, i, fp]),
+        "orc_proxy_geometry": (i, [fp, fp, fp, fp, i, fp, ctypes.POINTER(ctypes.c_int16),
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -227,3 +232,45 @@ class Oracle:
             for k, name in enumerate("uvw"):
                 out[f"{name}{h}"] = self.get_velocity(k, h)
         return out
+
+
+# ------------------------------------------------ proxy-plane slicing geometry
+
+PROXY_MAX_VERTS_PER_PLANE = 13
+PROXY_MAX_INDICES_PER_PLANE = 36
+
+
+def _f3(v):
+    a = np.ascontiguousarray(v, dtype=np.float32).reshape(3)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def plane_box(box_pos, box_size, plane_pos, plane_normal, sort=False):
+    """planeBoxIntersections (and optionally sortPoints): float32 (n, 3), the
+    mean last."""
+    lib = load()
+    keep = [_f3(v) for v in (box_pos, box_size, plane_pos, plane_normal)]
+    out = np.empty((13, 3), np.float32)
+    n = lib.orc_plane_box(*(k[1] for k in keep), int(bool(sort)),
+                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out[:n].copy()
+
+
+def proxy_geometry(box_pos, box_dims, view_origin, view_dir, steps):
+    """generateProxyGeometry: (vertices float32 (n, 3), indices int16 (m,),
+    planes int32 (steps, 2) of (num_triangles, triangle_offset))."""
+    lib = load()
+    keep = [_f3(v) for v in (box_pos, box_dims, view_origin, view_dir)]
+    steps = int(steps)
+    verts = np.empty((max(steps, 1) * PROXY_MAX_VERTS_PER_PLANE, 3), np.float32)
+    idx = np.empty(max(steps, 1) * PROXY_MAX_INDICES_PER_PLANE, np.int16)
+    planes = np.empty((max(steps, 1), 2), np.int32)
+    nv, ni = ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib.orc_proxy_geometry(*(k[1] for k in keep), steps,
+                                verts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+                                planes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                ctypes.byref(nv), ctypes.byref(ni))
+    if rc != 0:
+        raise ValueError("proxy_geometry: steps must be >= 1")
+    return verts[:nv.value].copy(), idx[:ni.value].copy(), planes[:steps].copy()

@@ -190,6 +190,14 @@ hipError_t obstacle_velocity_bc(const Grid& g, float* u, float* v, float* w, con
 hipError_t obstacle_velocity_bc(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* owner,
                                 const SphereList& vel, hipStream_t st);
 
+// Proxy-plane slicing geometry (proxy_kernels.hip): one workgroup builds
+// every plane of one view.
+struct ProxyView {
+    float box_pos[3], box_dims[3], origin[3], dir[3];
+};
+hipError_t proxy_geometry(const ProxyView& view, int steps, float* verts, int16_t* indices,
+                          smoke_proxy_plane* planes, int* counts, hipStream_t st);
+
 // Vorticity confinement on [now], after the force stage (an extension; three
 // passes over four FP32 padded scratch fields).
 struct VortBufs {

@@ -69,6 +69,15 @@ class JacobiPlan(ctypes.Structure):
 
 TILE_32x32, TILE_64x16 = 0, 1
 
+
+class ProxyPlane(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_int), ("triangle_offset", ctypes.c_int)]
+
+
+PROXY_MAX_STEPS = 4096
+PROXY_VERTICES_PER_PLANE = 13
+PROXY_INDICES_PER_PLANE = 36
+
 _lib = None
 
 
@@ -145,6 +154,9 @@ def _declare(lib):
         "smoke_server_result": (st, [vp, ctypes.c_longlong, ip, fp, dp, dp]),
         "smoke_server_stats_get": (st, [vp, ctypes.POINTER(ServerStats)]),
         "smoke_server_tick_ms": (st, [vp, dp, ip, i, ip]),
+        "smoke_proxy_geometry": (st, [fp, fp, fp, fp, i, fp, ctypes.POINTER(ctypes.c_int16),
+                                      ctypes.POINTER(ProxyPlane), ip, ip]),
+        "smoke_proxy_geometry_device": (st, [fp, fp, fp, fp, i, vp, vp, vp, vp, vp]),
         "smoke_legacy_device_info": (st, []),
         "smoke_legacy_initialize": (st, [fp, u, u, u]),
         "smoke_legacy_delete": (st, []),
@@ -619,3 +631,37 @@ class Server:
         _check(self.lib.smoke_server_tick_ms(self.h, ms, req, cap, ctypes.byref(n)))
         return (np.array(ms[: n.value], dtype=np.float64),
                 np.array(req[: n.value], dtype=np.int64))
+
+
+def _vec3(v):
+    a = np.ascontiguousarray(v, dtype=np.float32).reshape(3)
+    return a, _fptr(a)
+
+
+def proxy_geometry(box_position, box_dimensions, view_origin, view_direction, steps):
+    """Proxy-plane slicing geometry of one view (smoke_proxy_geometry):
+    (vertices float32 (n, 3), indices int16 (m,), planes int32 (steps, 2) of
+    (num_triangles, triangle_offset))."""
+    lib = load()
+    keep = [_vec3(v) for v in (box_position, box_dimensions, view_origin, view_direction)]
+    steps = int(steps)
+    cap = max(steps, 1)
+    verts = np.empty((cap * PROXY_VERTICES_PER_PLANE, 3), np.float32)
+    idx = np.empty(cap * PROXY_INDICES_PER_PLANE, np.int16)
+    planes = np.empty((cap, 2), np.int32)
+    nv, ni = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib.smoke_proxy_geometry(*(k[1] for k in keep), steps, _fptr(verts),
+                                    idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+                                    planes.ctypes.data_as(ctypes.POINTER(ProxyPlane)),
+                                    ctypes.byref(nv), ctypes.byref(ni)))
+    return verts[:nv.value].copy(), idx[:ni.value].copy(), planes[:steps].copy()
+
+
+def proxy_geometry_device(box_position, box_dimensions, view_origin, view_direction, steps,
+                          vertices, indices, planes, counts, stream=None):
+    """smoke_proxy_geometry_device on device pointers (ints), enqueued on
+    `stream` (a hipStream_t as int, None = the default stream)."""
+    lib = load()
+    keep = [_vec3(v) for v in (box_position, box_dimensions, view_origin, view_direction)]
+    _check(lib.smoke_proxy_geometry_device(*(k[1] for k in keep), int(steps), vertices, indices,
+                                           planes, counts, stream))
