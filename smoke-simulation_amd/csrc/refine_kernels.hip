@@ -124,8 +124,11 @@ __global__ __launch_bounds__(BX * BY) void k_ref_norm0(Grid g, const float* __re
 // next negated residual r = (d + acc*x_new) - sum_n x_new_n on active cells,
 // with its partial norms; LAST skips the r store.  Neighbour values are
 // recomputed with the same two operations, so they equal the neighbours' own
-// x_new.  A thread walks kZB planes of one column keeping the column's
-// values in registers.  Once `done` is set the pass only copies x_old.
+// x_new.  A thread walks kZB planes of one column: every load of the column
+// and of its x/y neighbours is issued first (plane indices clamped into the
+// padded field, so none is predicated), then the planes are computed and the
+// stores issued last, so the loads stay in flight together.  Once `done` is
+// set the pass only copies x_old.
 template <bool XZERO, bool LAST>
 __global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __restrict__ xo,
                                                        float* __restrict__ xn,
@@ -152,28 +155,54 @@ __global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __re
             for (int k = 0; k < kZB; ++k)
                 if (k < nk) xn[i0 + k * g.plane] = XZERO ? 0.0f : xo[i0 + k * g.plane];
         } else {
-            auto X = [&](int j) { return (XZERO ? 0.0f : xo[j]) + h2f(e[j]) * sinv; };
-            // Column values at planes z0-1 .. z0+kZB (clamped reads stay inside
-            // the padded field: planes 0..nz exist).
+            const int row = g.idx(x, y, 0);
+            auto plane_of = [&](int z) { return row + min(z, g.nz) * g.plane; };
+            // Column x_old and e at planes z0-1 .. z0+kZB, neighbours at z0 .. z0+kZB-1.
+            float cxo[kZB + 2], ce[kZB + 2];
+#pragma unroll
+            for (int k = 0; k < kZB + 2; ++k) {
+                const int j = plane_of(z0 - 1 + k);
+                cxo[k] = XZERO ? 0.0f : xo[j];
+                ce[k] = h2f(e[j]);
+            }
+            float nxo[kZB][4], ne[kZB][4], dd[kZB];
+            uint8_t fl[kZB];
+#pragma unroll
+            for (int k = 0; k < kZB; ++k) {
+                const int i = plane_of(z0 + k);
+                const int nb[4] = {i - 1, i + 1, i - g.px, i + g.px};
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    nxo[k][n] = XZERO ? 0.0f : xo[nb[n]];
+                    ne[k][n] = h2f(e[nb[n]]);
+                }
+                dd[k] = d[i];
+                fl[k] = flags[i];
+            }
             float col[kZB + 2];
 #pragma unroll
-            for (int k = 0; k < kZB + 2; ++k) col[k] = (k <= nk + 1) ? X(i0 + (k - 1) * g.plane) : 0.0f;
+            for (int k = 0; k < kZB + 2; ++k) col[k] = cxo[k] + ce[k] * sinv;
+            float rr[kZB];
+#pragma unroll
+            for (int k = 0; k < kZB; ++k) {
+                rr[k] = 0.0f;
+                if (k < nk && (fl[k] & F_ACTIVE)) {
+                    float X[4];
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) X[n] = nxo[k][n] + ne[k][n] * sinv;
+                    const float sum = dev::sum6(X[0], X[1], X[2], X[3], col[k], col[k + 2]);
+                    const float r = (dd[k] + (float)__popc(fl[k] & F_NBRS) * col[k + 1]) - sum;
+                    rr[k] = r;
+                    s2 += (double)r * (double)r;
+                    m = fmaxf(m, fabsf(r));
+                }
+            }
 #pragma unroll
             for (int k = 0; k < kZB; ++k) {
                 if (k >= nk) break;
                 const int i = i0 + k * g.plane;
-                const float xc = col[k + 1];
-                xn[i] = xc;
-                const uint8_t fl = flags[i];
-                float r = 0.0f;
-                if (fl & F_ACTIVE) {
-                    const float sum = dev::sum6(X(i - 1), X(i + 1), X(i - g.px), X(i + g.px), col[k],
-                                                col[k + 2]);
-                    r = (d[i] + (float)__popc(fl & F_NBRS) * xc) - sum;
-                    s2 += (double)r * (double)r;
-                    m = fmaxf(m, fabsf(r));
-                }
-                if (!LAST) q[i] = r;
+                xn[i] = col[k + 1];
+                if (!LAST) q[i] = rr[k];
             }
         }
     }
@@ -329,28 +358,32 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
 #pragma unroll
     for (int t = 0; t <= T; ++t) iq[t] = qq[t] = zero2;
 
-    // Prefetch ring (plane k + j in slot j), dev::kPrefetch planes deep.
+    // Prefetch ring (plane k + j in slot j), dev::kPrefetch planes deep.  The
+    // ring holds loaded values only (FIRST: the raw FP32 r); anything computed
+    // from them is computed when the plane is consumed, so no wait is forced
+    // at the load.
     constexpr int D = dev::kPrefetch;
     h2 ne[D], ni[D], nq[D];
-    auto load = [&](int k, h2& e_, h2& i_, h2& q_) {
-        e_ = zero2;
-        i_ = zero2;
-        q_ = zero2;
-        if (!(in_grid && k >= 0 && k <= g.nz && k < z1 + T)) return;
-        const int wi = wbase + k * wplane;
+    float2 nr[D];
+    // Loads are unconditional (plane index clamped into the field, lanes
+    // outside the grid read row 0); planes outside [0, nz] or past the chunk
+    // read as 0 when consumed.
+    auto load = [&](int k, h2& e_, h2& i_, h2& q_, float2& r_) {
+        const int wi = wbase + min(max(k, 0), g.nz) * wplane;
         i_ = ld_h2(inv, wi);
         if (FIRST) {
-            const float2 rq = *reinterpret_cast<const float2*>(q + 2 * wi);
-            q_.x = (_Float16)(rq.x * s * (float)i_.x);
-            q_.y = (_Float16)(rq.y * s * (float)i_.y);
-            if (writes && k >= z0 && k < z1) st_h2(qi, wi, q_);
+            e_ = zero2;
+            q_ = zero2;
+            r_ = *reinterpret_cast<const float2*>(q + 2 * wi);
         } else {
             e_ = ld_h2(ein, wi);
             q_ = ld_h2(qi, wi);
+            r_ = make_float2(0.0f, 0.0f);
         }
     };
+    auto valid = [&](int k) { return in_grid && k >= 0 && k <= g.nz && k < z1 + T; };
 #pragma unroll
-    for (int j = 0; j < D; ++j) load(z0 - T + j, ne[j], ni[j], nq[j]);
+    for (int j = 0; j < D; ++j) load(z0 - T + j, ne[j], ni[j], nq[j], nr[j]);
     // Unrolled by lcm(3, T+1) so the register windows rotate without copies;
     // the padded iterations load and write nothing.
     constexpr int U = dev::z_unroll(T);
@@ -359,14 +392,22 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int k = kb + u;
-        const h2 p0 = ne[0], ii = ni[0], qv = nq[0];
+        const bool ok = valid(k);
+        const h2 p0 = ok ? ne[0] : zero2, ii = ok ? ni[0] : zero2;
+        h2 qv = ok ? nq[0] : zero2;
+        if (FIRST) {
+            const float2 rv = ok ? nr[0] : make_float2(0.0f, 0.0f);
+            qv.x = (_Float16)(rv.x * s * (float)ii.x);
+            qv.y = (_Float16)(rv.y * s * (float)ii.y);
+        }
 #pragma unroll
         for (int j = 0; j + 1 < D; ++j) {
             ne[j] = ne[j + 1];
             ni[j] = ni[j + 1];
             nq[j] = nq[j + 1];
+            nr[j] = nr[j + 1];
         }
-        load(k + D, ne[D - 1], ni[D - 1], nq[D - 1]);
+        load(k + D, ne[D - 1], ni[D - 1], nq[D - 1], nr[D - 1]);
 #pragma unroll
         for (int t = T; t > 0; --t) {
             iq[t] = iq[t - 1];
@@ -405,6 +446,10 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
         }
         const int zo = k - T;
         if (writes && zo >= z0 && zo < z1) st_h2(eout, wbase + zo * wplane, newv[T]);
+        // The scaled right-hand side for the later launches of the refinement,
+        // stored as its plane is consumed: a store issued among the prefetch
+        // loads would make every later wait drain all outstanding loads.
+        if (FIRST && writes && k >= z0 && k < z1) st_h2(qi, wbase + k * wplane, qv);
 #pragma unroll
         for (int t = 0; t < T; ++t)
             sh.v[t][cur][ty + 1][tx + 1] = __builtin_bit_cast(uint32_t, newv[t]);

@@ -273,15 +273,17 @@ __device__ __forceinline__ void jacobi_at(const Grid& g, const float* __restrict
     pout[i] = (inv > 0.0f) ? out : 0.0f;
 }
 
-// Unroll factor of the z-streaming loops of the fused sweeps: the register
-// windows rotate with period 3 (z neighbours) and T+1 (per-plane diagonal
-// and right-hand side), so after lcm(3, T+1) iterations every register is
-// back in its place and the unrolled body needs no copies.
-__host__ __device__ constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
-__host__ __device__ constexpr int z_unroll(int T) { return 3 * (T + 1) / gcd_c(3, T + 1); }
-// Planes in flight per thread of the z-streaming loops (a divisor of every
-// z_unroll, so the prefetch ring rotates without copies as well).
+// Planes in flight per thread of the z-streaming loops of the fused sweeps:
+// one loop iteration (one plane, one barrier) is much shorter than the HBM
+// latency under load, so loads are issued kPrefetch iterations ahead.
 constexpr int kPrefetch = 3;
+// Unroll factor of those loops: the register windows rotate with period 3 (z
+// neighbours), T+1 (per-plane diagonal and right-hand side) and kPrefetch
+// (the prefetch ring), so after the lcm of the three every register is back
+// in its place and the unrolled body needs no copies.
+__host__ __device__ constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+__host__ __device__ constexpr int lcm_c(int a, int b) { return a / gcd_c(a, b) * b; }
+__host__ __device__ constexpr int z_unroll(int T) { return lcm_c(lcm_c(3, T + 1), kPrefetch); }
 
 // The value held by the previous / next lane of the wave64 (DPP wave_shr:1 /
 // wave_shl:1 row moves); lane 0 / lane 63 receive 0.  The fused sweeps take
@@ -352,20 +354,19 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
 #pragma unroll
     for (int t = 0; t <= T; ++t) iq[t] = dq[t] = 0.0f;
 
-    // Prefetch ring: plane k + j in slot j.
+    // Prefetch ring: plane k + j in slot j.  The loads are unconditional (the
+    // plane index is clamped into the field, and lanes outside the grid read
+    // row 0), so nothing waits on them until the plane is consumed; values of
+    // planes outside [0, nz) or past the chunk are replaced by 0 there.
     float np[kPrefetch], nd[kPrefetch];
     uint8_t nf[kPrefetch];
     auto load = [&](int kn, float& p, float& dv, uint8_t& f) {
-        p = 0.0f;
-        dv = 0.0f;
-        f = 0;
-        if (in_grid && kn >= 0 && kn < g.nz && kn < z1 + T) {
-            const int i = base + kn * g.plane;
-            p = zero_in ? 0.0f : pin[i];
-            f = flags[i];
-            dv = d[i];
-        }
+        const int i = base + min(max(kn, 0), g.nz - 1) * g.plane;
+        p = zero_in ? 0.0f : pin[i];
+        f = flags[i];
+        dv = d[i];
     };
+    auto valid = [&](int kn) { return in_grid && kn >= 0 && kn < g.nz && kn < z1 + T; };
 #pragma unroll
     for (int j = 0; j < kPrefetch; ++j) load(z0 - T + j, np[j], nd[j], nf[j]);
     // The trip count is rounded up to a multiple of the unroll factor; the
@@ -377,8 +378,9 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int k = kb + u;
-        const float p0 = np[0], dd = nd[0];
-        const uint8_t fl = nf[0];
+        const bool ok = valid(k);
+        const float p0 = ok ? np[0] : 0.0f, dd = ok ? nd[0] : 0.0f;
+        const uint8_t fl = ok ? nf[0] : (uint8_t)0;
 #pragma unroll
         for (int j = 0; j + 1 < kPrefetch; ++j) {
             np[j] = np[j + 1];
