@@ -12,7 +12,9 @@ enqueued back to back and bracketed by stream synchronisations.
 Other workloads: grid_512 (512^3, 8 sources), reference_80 (the reference's
 own 80^3 scene, red-black SOR on faces), batch_64 (64 simulations of uneven
 sizes in [32,160]^3, per-simulation dt, Jacobi x60, one launch per stage,
-replayed from a HIP graph), serve (a Poisson stream of requests, grids
+replayed from a HIP graph), proxy_80 (the renderer's proxy-plane slicing
+geometry of an 80^3 box, 100 planes per view, on the GPU; metric planes/s),
+serve (a Poisson stream of requests, grids
 log-uniform in 48..256 per axis, 20-200 steps each, arriving at 80 % of the
 estimated capacity, through the continuous-batching server under an HBM
 budget; reports request-step latency p50/p99).
@@ -276,7 +278,9 @@ def run_single(args, sc):
         with open(traffic_file) as f:
             tr = json.load(f)
         key = f"{sc['name']}:{roofline['kernel']}"
-        if key in tr:
+        # Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+        # calibrated), valid only for the launch plan they were measured with.
+        if key in tr and tr[key].get("plan") == roofline["jacobi_plan"]:
             roofline["traffic"] = tr[key]["bytes_per_launch"]
 
     step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
@@ -436,13 +440,121 @@ def run_serve(args):
     return out
 
 
+def proxy_view_80():
+    """The renderer's default view: an 80^3 volume of unit cells at the origin,
+    sliced along the half vector between the camera at (1, 20, 100) and the
+    light at (120, 80, 0) (project/main.cpp:87-88, :133, :600, :676-686)."""
+    import numpy as np
+    f = np.float32
+
+    def normalize(v):
+        v = np.asarray(v, f)
+        return v * (f(1) / np.sqrt(np.dot(v, v), dtype=f))
+
+    cam = normalize(-np.array([1.0, 20.0, 100.0], f))
+    light = normalize(-np.array([120.0, 80.0, 0.0], f))
+    half = cam + light
+    if np.dot(cam, light) < 0:
+        half = -cam + light
+    return (0.0, 0.0, 0.0), (80.0, 80.0, 80.0), (0.0, 0.0, 0.0), tuple(float(t) for t in -half)
+
+
+def run_proxy(args):
+    """Proxy-plane slicing geometry of the renderer's default view, planes per
+    view as the app's default (100): GPU kernel time per view (HIP events over
+    back-to-back device launches), the host entry point's time per view
+    (launch, read-back of the arrays into host memory), and the oracle's
+    sequential CPU time as the baseline."""
+    import ctypes
+    import numpy as np
+    import smokesim
+    planes = args.planes
+    view = proxy_view_80()
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipFree.argtypes = [vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    sizes = [planes * smokesim.PROXY_VERTICES_PER_PLANE * 12,
+             planes * smokesim.PROXY_INDICES_PER_PLANE * 2, planes * 8, 8]
+    ptrs = []
+    for n in sizes:
+        p = vp()
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise SystemExit("hipMalloc failed")
+        ptrs.append(p.value)
+    ev = [vp(), vp()]
+    for e in ev:
+        hip.hipEventCreate(ctypes.byref(e))
+    for _ in range(args.warmup):
+        smokesim.proxy_geometry_device(*view, planes, *ptrs, None)
+    hip.hipEventRecord(ev[0], None)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        smokesim.proxy_geometry_device(*view, planes, *ptrs, None)
+    hip.hipEventRecord(ev[1], None)
+    hip.hipEventSynchronize(ev[1])
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1])
+    kernel_us = 1000.0 * ms.value / args.steps
+    for e in ev:
+        hip.hipEventDestroy(e)
+    for p in ptrs:
+        hip.hipFree(p)
+    # Host entry point: the per-frame cost a renderer sees with host arrays.
+    verts, idx, tbl = smokesim.proxy_geometry(*view, planes)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        smokesim.proxy_geometry(*view, planes)
+    host_us = 1e6 * (time.perf_counter() - t0) / args.steps
+    out_bytes = verts.nbytes + idx.nbytes + tbl.nbytes
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        rv, ri, rp = orc.proxy_geometry(*view, planes)
+        if not (np.array_equal(ri, idx) and np.array_equal(rp, tbl)
+                and np.array_equal(rv.view(np.uint32), verts.view(np.uint32))):
+            raise SystemExit("proxy geometry differs from the oracle")
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_budget, 5.0):
+            orc.proxy_geometry(*view, planes)
+            n += 1
+        cpu_s = (time.perf_counter() - t0) / n
+        cpu = {"value": round(planes / cpu_s, 1), "unit": "planes/s", "cores": 1, "kind": "port",
+               "sample": f"{n} views of {planes} planes, sequential FP32 oracle"}
+    achieved = out_bytes / (kernel_us * 1e-6) / 1e9
+    return {
+        "metric": "proxy planes/s", "value": round(planes / (kernel_us * 1e-6), 1),
+        "unit": "planes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(kernel_us / 1000.0, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"proxy_80: slicing geometry of an 80^3 box, {planes} planes per "
+                               "view, the renderer's default half-vector view",
+                   "planes": planes, "parallelism": "single-gpu"},
+        "host_entry_us_per_view": round(host_us, 2),
+        "launch_us_per_view_wall": round(1e6 * wall / args.steps, 2),
+        "roofline": {"bound": "hbm", "kernel": "k_proxy_geometry", "achieved": round(achieved, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": None, "avg_launch_us": round(kernel_us, 3),
+                     "alg_bytes_per_launch": int(out_bytes),
+                     "note": "one workgroup per view: launch- and latency-bound, not bandwidth-bound"},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="grid_256",
-                    choices=["grid_256", "grid_512", "reference_80", "batch_64", "serve"])
+                    choices=["grid_256", "grid_512", "reference_80", "batch_64", "serve", "proxy_80"])
     ap.add_argument("--projection", choices=["jacobi", "rbsor", "reference"], default=None)
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -459,6 +571,7 @@ def main():
                     help="serve: cell-updates/s the arrival rate is 80%% of")
     ap.add_argument("--rate", type=float, default=None, help="serve: requests/s (overrides)")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget, GiB")
+    ap.add_argument("--planes", type=int, default=100, help="proxy_80: planes per view")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -472,6 +585,8 @@ def main():
     log(smokesim.device_info())
     if args.workload == "batch_64":
         out = run_batch(args, scenes.batch_64())
+    elif args.workload == "proxy_80":
+        out = run_proxy(args)
     elif args.workload == "serve":
         print("bench.py: the serve workload is disabled: its first GPU test run aborted and the "
               "cause has not been found, so the serving loop does not run on the GPU",

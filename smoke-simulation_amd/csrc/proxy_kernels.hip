@@ -51,57 +51,97 @@ __device__ __forceinline__ float pseudoangle(float y, float x)
 
 // Edge points of one plane (12 at most: points of 0/0 edges are NaN and never
 // equal, so none of them is dropped) and their mean, last.
-constexpr int kMaxPoints = 13;
+constexpr int kEdges = 12;
+constexpr int kMaxPoints = kEdges + 1;
 struct Poly {
     V3 p[kMaxPoints];
     int n;
 };
 
-__device__ __forceinline__ void visit(Poly& P, V3& avg, V3 ro, V3 rd, float max_t, V3 po, V3 pn)
+// All 12 edge tests of planeBoxIntersections in the reference's order (x
+// edges over y then z, y edges over x then z, z edges over y then x), kept in
+// registers: q[e] and keep[e] (hit, and no earlier kept point equal to it).
+// The mean sums the kept points in that order, as the sequential appends do.
+__device__ __forceinline__ void plane_box(V3 bp, V3 bs, V3 po, V3 pn, V3 (&q)[kEdges],
+                                          bool (&keep)[kEdges], int& n, V3& mean)
 {
-    const float t = dot(sub(po, ro), pn) / dot(rd, pn);
-    if (t > max_t || t < 0) return;
-    const V3 q = add(ro, muls(rd, t));
-    for (int i = 0; i < P.n; ++i)
-        if (q.x == P.p[i].x && q.y == P.p[i].y && q.z == P.p[i].z) return;
-    P.p[P.n++] = q;
-    avg = add(avg, q);
-}
-
-__device__ void plane_box(Poly& P, V3 bp, V3 bs, V3 po, V3 pn)
-{
-    P.n = 0;
     const V3 h = divs(bs, 2.0f);
+#pragma unroll
+    for (int e = 0; e < kEdges; ++e) {
+        const int axis = e >> 2;
+        const float a = (e & 2) ? 1.0f : -1.0f, b = (e & 1) ? 1.0f : -1.0f;
+        V3 off, rd;
+        float max_t;
+        if (axis == 0) {
+            off = V3{h.x, h.y * a, h.z * b};
+            rd = V3{1, 0, 0};
+            max_t = bs.x;
+        } else if (axis == 1) {
+            off = V3{h.x * a, h.y, h.z * b};
+            rd = V3{0, 1, 0};
+            max_t = bs.y;
+        } else {
+            off = V3{h.x * b, h.y * a, h.z};
+            rd = V3{0, 0, 1};
+            max_t = bs.z;
+        }
+        const V3 ro = sub(bp, off);
+        const float t = dot(sub(po, ro), pn) / dot(rd, pn);
+        keep[e] = !(t > max_t || t < 0);
+        q[e] = add(ro, muls(rd, t));
+    }
     V3 avg{0.0f, 0.0f, 0.0f};
-    for (int y = -1; y < 2; y += 2)
-        for (int z = -1; z < 2; z += 2)
-            visit(P, avg, sub(bp, V3{h.x, h.y * (float)y, h.z * (float)z}), V3{1, 0, 0}, bs.x, po, pn);
-    for (int x = -1; x < 2; x += 2)
-        for (int z = -1; z < 2; z += 2)
-            visit(P, avg, sub(bp, V3{h.x * (float)x, h.y, h.z * (float)z}), V3{0, 1, 0}, bs.y, po, pn);
-    for (int y = -1; y < 2; y += 2)
-        for (int x = -1; x < 2; x += 2)
-            visit(P, avg, sub(bp, V3{h.x * (float)x, h.y * (float)y, h.z}), V3{0, 0, 1}, bs.z, po, pn);
-    P.p[P.n] = divs(avg, (float)P.n);
-    P.n += 1;
+    n = 0;
+#pragma unroll
+    for (int j = 0; j < kEdges; ++j) {
+        bool dup = false;
+#pragma unroll
+        for (int i = 0; i < j; ++i)
+            dup = dup || (keep[i] && q[j].x == q[i].x && q[j].y == q[i].y && q[j].z == q[i].z);
+        keep[j] = keep[j] && !dup;
+        if (keep[j]) {
+            avg = add(avg, q[j]);
+            ++n;
+        }
+    }
+    mean = divs(avg, (float)n);
 }
 
-// Descending pseudoangle about the mean in the frame (ax, normal x ax), ax
-// toward the first point found; the insertion sort keeps equal keys in
-// discovery order (it is the algorithm libstdc++'s std::sort applies to
-// ranges this short).
-__device__ void sort_points(Poly& P, V3 normal)
+// Pseudoangle sort keys about the mean in the frame (ax, normal x ax), ax
+// toward the first kept point.  Returns false if a key is NaN.
+__device__ __forceinline__ bool sort_keys(const V3 (&q)[kEdges], const bool (&keep)[kEdges], V3 o,
+                                          V3 normal, float (&key)[kEdges])
+{
+    normal = normalize(normal);
+    V3 first = o;
+    bool found = false;
+#pragma unroll
+    for (int j = kEdges - 1; j >= 0; --j)
+        if (keep[j]) {
+            first = q[j];
+            found = true;
+        }
+    const V3 ax = normalize(sub(first, o));
+    const V3 ay = cross(normal, ax);
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < kEdges; ++j) {
+        const V3 v = sub(q[j], o);
+        key[j] = pseudoangle(dot(ay, v), dot(ax, v));
+        ok = ok && (!keep[j] || key[j] == key[j]);
+    }
+    return ok || !found;
+}
+
+// The reference sorts with std::sort, which for ranges this short is
+// libstdc++'s insertion sort: a stable sort, here by descending key.  With
+// keys that are all numbers (the regular case) each kept point's place is its
+// rank: the kept points with a larger key plus the earlier kept points with
+// an equal one.  NaN keys (a plane through an edge lying in it, or a plane
+// that meets the box in a single point) take the insertion sort itself.
+__device__ void insertion_sort(Poly& P, float (&key)[kMaxPoints])
 {
     const int m = P.n - 1;
-    const V3 o = P.p[m];
-    normal = normalize(normal);
-    const V3 ax = normalize(sub(P.p[0], o));
-    const V3 ay = cross(normal, ax);
-    float key[kMaxPoints];
-    for (int i = 0; i < m; ++i) {
-        const V3 v = sub(P.p[i], o);
-        key[i] = pseudoangle(dot(ay, v), dot(ax, v));
-    }
     for (int i = 1; i < m; ++i) {
         const V3 pv = P.p[i];
         const float kv = key[i];
@@ -153,17 +193,23 @@ __global__ __launch_bounds__(kProxyThreads) void k_proxy_geometry(
     int vbase = 0, ibase = 0;
     for (int c0 = 0; c0 < steps; c0 += kProxyThreads) {
         const int k = c0 + tid;
-        Poly P;
-        P.n = 0;
-        int nv = 0, ni = 0;
+        V3 q[kEdges];
+        bool keep[kEdges];
+        float key[kEdges];
+        V3 mean{0.0f, 0.0f, 0.0f};
+        int npts = 0;
+        bool ranked = true;
         if (k < steps) {
             const float t = (0.5f + (float)k) / (float)steps;
             const V3 pos = add(start, muls(sub(end, start), t));
-            plane_box(P, bp, dims, pos, dir);
-            sort_points(P, dir);
-            nv = P.n;
-            ni = P.n >= 2 ? 3 * (P.n - 1) : 0;
+            plane_box(bp, dims, pos, dir, q, keep, npts, mean);
+            ranked = sort_keys(q, keep, mean, dir, key);
+        } else {
+#pragma unroll
+            for (int e = 0; e < kEdges; ++e) keep[e] = false;
         }
+        const int nv = k < steps ? npts + 1 : 0;
+        const int ni = k < steps && npts >= 1 ? 3 * npts : 0;
         // Inclusive prefix sums of the counts over the workgroup.
         scan_v[tid] = nv;
         scan_i[tid] = ni;
@@ -179,26 +225,53 @@ __global__ __launch_bounds__(kProxyThreads) void k_proxy_geometry(
         if (k < steps) {
             const int vo = vbase + scan_v[tid] - nv;
             const int io = ibase + scan_i[tid] - ni;
-            for (int i = 0; i < P.n; ++i) {
-                verts[3 * (vo + i) + 0] = P.p[i].x;
-                verts[3 * (vo + i) + 1] = P.p[i].y;
-                verts[3 * (vo + i) + 2] = P.p[i].z;
+            if (ranked) {
+#pragma unroll
+                for (int j = 0; j < kEdges; ++j) {
+                    if (!keep[j]) continue;
+                    int slot = 0;
+#pragma unroll
+                    for (int i = 0; i < kEdges; ++i)
+                        slot += keep[i] && (key[i] > key[j] || (i < j && key[i] == key[j]));
+                    verts[3 * (vo + slot) + 0] = q[j].x;
+                    verts[3 * (vo + slot) + 1] = q[j].y;
+                    verts[3 * (vo + slot) + 2] = q[j].z;
+                }
+            } else {
+                Poly P;
+                float pk[kMaxPoints];
+                P.n = 0;
+                for (int j = 0; j < kEdges; ++j)
+                    if (keep[j]) {
+                        pk[P.n] = key[j];
+                        P.p[P.n++] = q[j];
+                    }
+                P.p[P.n++] = mean;
+                insertion_sort(P, pk);
+                for (int i = 0; i + 1 < P.n; ++i) {
+                    verts[3 * (vo + i) + 0] = P.p[i].x;
+                    verts[3 * (vo + i) + 1] = P.p[i].y;
+                    verts[3 * (vo + i) + 2] = P.p[i].z;
+                }
             }
+            verts[3 * (vo + npts) + 0] = mean.x;
+            verts[3 * (vo + npts) + 1] = mean.y;
+            verts[3 * (vo + npts) + 2] = mean.z;
             // teselate: fan about the mean, closed by (mean, last, first);
             // the reference stores 16-bit indices (values wrap past 32767).
-            const int centre = vo + P.n - 1;
+            const int centre = vo + npts;
             int w = io;
-            if (P.n >= 2) {
-                for (int i = 0; i < P.n - 2; ++i) {
+            if (npts >= 1) {
+                for (int i = 0; i < npts - 1; ++i) {
                     indices[w++] = (int16_t)(uint16_t)centre;
                     indices[w++] = (int16_t)(uint16_t)(vo + i);
                     indices[w++] = (int16_t)(uint16_t)(vo + i + 1);
                 }
                 indices[w++] = (int16_t)(uint16_t)centre;
-                indices[w++] = (int16_t)(uint16_t)(vo + P.n - 2);
+                indices[w++] = (int16_t)(uint16_t)(vo + npts - 1);
                 indices[w++] = (int16_t)(uint16_t)vo;
             }
-            planes[k].num_triangles = P.n - 1;
+            planes[k].num_triangles = npts;
             planes[k].triangle_offset = io / 3;
         }
         vbase += scan_v[kProxyThreads - 1];
