@@ -42,6 +42,9 @@ struct smoke_batch {
     std::vector<hipEvent_t> pool;
     std::vector<double> step_ms;
     bool broken = false;  // a member simulation was destroyed
+    // Largest z-chunk of the fused sweeps every member's planes allow (the
+    // 32-bit buffer span of a block, fused_max_zc); plans are capped to it.
+    int zc_cap = 0;
     // Resident workgroups per CU (queried outside graph capture).
     int occ_cells = 1;
     int occ_jacobi[7] = {1, 1, 1, 1, 1, 1, 1};  // by sweeps per launch
@@ -246,6 +249,8 @@ smoke_status smoke_batch_create(smoke_sim* const* sims, int count, const float* 
         int ns = 0;
         for (const smk::Object& o : s->objects) ns += (o.type == 1);
         if (ns > 8) return fail(SMOKE_ERR_INVALID, "batch_create: simulation %d has more than 8 sources", i);
+        if (smk::fused_max_zc(s->g.plane) < 1)
+            return fail(SMOKE_ERR_INVALID, "batch_create: simulation %d's planes are too large to batch", i);
         sources += ns;
     }
     smoke_batch* b = new smoke_batch();
@@ -255,7 +260,10 @@ smoke_status smoke_batch_create(smoke_sim* const* sims, int count, const float* 
     b->iterations = sims[0]->prm.iterations;
     b->omega = (float)sims[0]->prm.jacobi_omega;
     b->plain = sims[0]->prm.jacobi_omega == 1.0 ? 1 : 0;
+    b->zc_cap = 1 << 30;
+    for (int i = 0; i < count; ++i) b->zc_cap = std::min(b->zc_cap, smk::fused_max_zc(sims[i]->g.plane));
     b->plan = sims[0]->plan;
+    b->plan.zc = std::min(b->plan.zc, b->zc_cap);
     auto bail = [&](smoke_status st) {
         smoke_batch_destroy(b);
         return st;
@@ -432,7 +440,7 @@ smoke_status smoke_batch_set_jacobi_plan(smoke_batch* b, const smoke_jacobi_plan
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->plan.sweeps_per_launch = plan->sweeps_per_launch;
     b->plan.tile = plan->tile;
-    b->plan.zc = plan->zc;
+    b->plan.zc = std::min(plan->zc, b->zc_cap);
     drop_graph(b);
     HIP_TRY(rebuild_jacobi_table(b));
     return SMOKE_OK;
@@ -466,7 +474,7 @@ smoke_status smoke_batch_autotune(smoke_batch* b, smoke_jacobi_plan* chosen, dou
             for (int zc : zcs) {
                 b->plan.sweeps_per_launch = t;
                 b->plan.tile = tile;
-                b->plan.zc = zc;
+                b->plan.zc = std::min(zc, b->zc_cap);
                 e = rebuild_jacobi_table(b);
                 int L = 0;
                 if (e == hipSuccess) e = enqueue_sweeps(b, b->plan, &L);  // warm-up

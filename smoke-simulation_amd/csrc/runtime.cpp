@@ -343,8 +343,11 @@ hipError_t enqueue_jacobi(smoke_sim* s, const smk::JacobiPlan& plan, int sweeps,
                           long long* launches)
 {
     const bool plain = s->prm.jacobi_omega == 1.0;
+    // A plane too large for one fused block's 32-bit buffer span: one sweep
+    // per launch (same bits).
+    const int per_launch = smk::fused_max_zc(s->g.plane) < 1 ? 1 : plan.sweeps_per_launch;
     for (int left = sweeps; left > 0;) {
-        const int t = std::min(left, plan.sweeps_per_launch);
+        const int t = std::min(left, per_launch);
         const int out = 1 - s->pcur;
         hipError_t e = smk::jacobi_fused(s->g, s->P[s->pcur], s->P[out], s->d, s->flags, plan, t,
                                          (float)s->prm.jacobi_omega, plain ? 1 : 0, zero_in ? 1 : 0,

@@ -79,6 +79,17 @@ struct ForceArgs {
 // Launch plan of the fused Jacobi kernel: sweeps fused per launch (2..6),
 // thread-block tile and z-chunk length.  Every plan gives bit-identical
 // results; the choice only changes speed.
+// The fused sweeps address one block's span of planes with 32-bit buffer
+// offsets below 2^31 bytes: a z-chunk of zc planes plus at most
+// kFusedSlackPlanes more.  Largest usable zc for a plane of `plane` elements
+// (< 1: no fused launch fits; the sweeps then run one per launch).
+constexpr int kFusedSlackPlanes = 48;
+__host__ __device__ inline int fused_max_zc(long long plane)
+{
+    const long long planes = 2147483647LL / (plane * 4LL);
+    return (int)(planes > 1000000 ? 1000000 : planes) - kFusedSlackPlanes;
+}
+
 struct JacobiPlan {
     enum Tile { TILE_32x32 = 0, TILE_64x16 = 1 };
     int sweeps_per_launch = 4;

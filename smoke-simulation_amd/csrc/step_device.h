@@ -309,11 +309,47 @@ __device__ __forceinline__ float lane_next(float v)
 
 // LDS of one fused-Jacobi block: per level, two planes of the grown tile,
 // padded by one on each side (edge threads read the never-written pad, which
-// only reaches halo columns that are not written out).
+// only reaches halo columns that are not written out; a level's plane is
+// contiguous, so the lanes of a row read consecutive banks); and the diagonal
+// inverse of every flags byte.
 template <int TX, int TY, int T>
 struct FusedLds {
     float v[T][2][TY + 2][TX + 2];
+    float inv[256];
 };
+
+// Raw buffer access (gfx9 buffer descriptor: a uniform base and a size in
+// bytes; a load at or past the size returns 0).  Offsets are 32-bit and
+// unsigned, so the address needs no per-lane 64-bit arithmetic, and an offset
+// pushed out of range turns a load into a 0 with no per-lane test.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+}
+__device__ __forceinline__ uint8_t buf_u8(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)byte_off, 0, 0);
+}
+__device__ __forceinline__ void buf_st_f32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)byte_off, 0, 0);
+}
+
+// Planes of slack a fused-sweep block's buffer offsets need beyond its z-chunk
+// of zc planes: T below and T above the chunk, the loop's padding to a whole
+// unroll (< z_unroll(T)) and the prefetch, plus T more so that the offsets of
+// lanes outside the grid (which start at 2^31) stay past the span.  The host
+// bounds zc with kFusedSlackPlanes (fused_max_zc, smoke_internal.h).
+__host__ __device__ constexpr int fused_slack_planes(int T) { return 3 * T + z_unroll(T) + kPrefetch + 1; }
+static_assert(fused_slack_planes(2) <= kFusedSlackPlanes && fused_slack_planes(3) <= kFusedSlackPlanes &&
+                  fused_slack_planes(4) <= kFusedSlackPlanes && fused_slack_planes(5) <= kFusedSlackPlanes &&
+                  fused_slack_planes(6) <= kFusedSlackPlanes,
+              "z-chunk slack of the fused sweeps");
 
 // T fused Jacobi sweeps (temporal blocking) for one tile (tile_x, tile_y) and
 // z-chunk tile_z of a TX x TY thread block.  The block computes the tile
@@ -322,12 +358,15 @@ struct FusedLds {
 // inside the grown tile.  Each thread holds one column and streams it in z:
 // level t (sweep t) at plane k-t is computed from level t-1 at planes
 // k-t-1, k-t, k-t+1 kept in registers and from the four x/y neighbours of
-// level t-1 at plane k-t held in LDS (double buffered, one barrier per plane).
-// Every value is computed with the single-sweep arithmetic, so results are
-// bit-identical to T single sweeps.  Planes are prefetched into registers
-// kPrefetch iterations ahead (one iteration is shorter than the HBM latency).
-// All threads of the block must call this with the same tile (the barriers
-// are block-wide).
+// level t-1 at plane k-t (x from the adjacent lanes, y from LDS, double
+// buffered, one barrier per plane).  Every value is computed with the
+// single-sweep arithmetic, so results are bit-identical to T single sweeps.
+// Planes are prefetched into registers kPrefetch iterations ahead.  The block
+// reads planes [plo, phi) of P, d and flags through buffer descriptors over
+// exactly that span: planes outside it and lanes outside the grid (offsets
+// from 2^31) read 0, so the loads need no validity selects.  Requires
+// zc <= fused_max_zc(plane, T).  All threads of the block must call this with
+// the same tile (the barriers are block-wide).
 template <int TX, int TY, int T, bool DAMPED>
 __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __restrict__ pin,
                                                   float* __restrict__ pout,
@@ -345,7 +384,19 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
     const bool in_grid = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
     const bool writes = tx >= T && tx < TX - T && ty >= T && ty < TY - T && x <= g.nx - 2 &&
                         y <= g.ny - 2;
-    const int base = in_grid ? g.idx(x, y, 0) : 0;
+    const int plo = max(z0 - T, 0), phi = min(z1 + T, g.nz);
+    const uint32_t plane = (uint32_t)g.plane;
+    const uint32_t span = (uint32_t)(phi - plo) * plane;  // elements
+    const size_t first = (size_t)plo * (size_t)g.plane;
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(pin + first, span * 4u);
+    const __amdgpu_buffer_rsrc_t rd = buf_rsrc(d + first, span * 4u);
+    const __amdgpu_buffer_rsrc_t rf = buf_rsrc(flags + first, span);
+    const __amdgpu_buffer_rsrc_t ro = buf_rsrc(pout + first, span * 4u);
+    const uint32_t cell = in_grid ? (uint32_t)g.idx(x, y, 0) : 0u;
+    const uint32_t lane_b = in_grid ? cell * 4u : 0x80000000u;  // byte offset in plane plo
+    const uint32_t lane_e = in_grid ? cell : 0x80000000u;       // element offset
+    for (int f = tx + TX * ty; f < 256; f += TX * TY) sh.inv[f] = cell_inv((uint8_t)f);
+    __syncthreads();
 
     float w[T][3];  // level t at planes (newest-2, newest-1, newest)
     float iq[T + 1], dq[T + 1];  // inverse and rhs of planes k, k-1, ..., k-T
@@ -354,33 +405,28 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
 #pragma unroll
     for (int t = 0; t <= T; ++t) iq[t] = dq[t] = 0.0f;
 
-    // Prefetch ring: plane k + j in slot j.  The loads are unconditional (the
-    // plane index is clamped into the field, and lanes outside the grid read
-    // row 0), so nothing waits on them until the plane is consumed; values of
-    // planes outside [0, nz) or past the chunk are replaced by 0 there.
+    // Prefetch ring: plane k + j in slot j.  Planes below plo wrap to offsets
+    // near 2^32 and planes at or past phi lie past the span: both read 0.
     float np[kPrefetch], nd[kPrefetch];
     uint8_t nf[kPrefetch];
     auto load = [&](int kn, float& p, float& dv, uint8_t& f) {
-        const int i = base + min(max(kn, 0), g.nz - 1) * g.plane;
-        p = zero_in ? 0.0f : pin[i];
-        f = flags[i];
-        dv = d[i];
+        const uint32_t dk = (uint32_t)(kn - plo) * plane;
+        p = zero_in ? 0.0f : buf_f32(rp, lane_b + dk * 4u);
+        dv = buf_f32(rd, lane_b + dk * 4u);
+        f = buf_u8(rf, lane_e + dk);
     };
-    auto valid = [&](int kn) { return in_grid && kn >= 0 && kn < g.nz && kn < z1 + T; };
 #pragma unroll
     for (int j = 0; j < kPrefetch; ++j) load(z0 - T + j, np[j], nd[j], nf[j]);
     // The trip count is rounded up to a multiple of the unroll factor; the
-    // extra iterations load nothing (kn >= z1 + T) and write nothing
-    // (zo >= z1).
+    // extra iterations load nothing in range and write nothing (zo >= z1).
     constexpr int U = z_unroll(T);
     const int kend = z0 - T + ((z1 - z0 + 2 * T + U - 1) / U) * U;
     for (int kb = z0 - T; kb < kend; kb += U)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int k = kb + u;
-        const bool ok = valid(k);
-        const float p0 = ok ? np[0] : 0.0f, dd = ok ? nd[0] : 0.0f;
-        const uint8_t fl = ok ? nf[0] : (uint8_t)0;
+        const float p0 = np[0], dd = nd[0];
+        const uint8_t fl = nf[0];
 #pragma unroll
         for (int j = 0; j + 1 < kPrefetch; ++j) {
             np[j] = np[j + 1];
@@ -393,7 +439,7 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
             iq[t] = iq[t - 1];
             dq[t] = dq[t - 1];
         }
-        iq[0] = cell_inv(fl);
+        iq[0] = sh.inv[fl];
         dq[0] = dd;
         w[0][0] = w[0][1];
         w[0][1] = w[0][2];
@@ -425,7 +471,8 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
             }
         }
         const int zo = k - T;
-        if (writes && zo >= z0 && zo < z1) pout[base + zo * g.plane] = newv[T];
+        if (writes && zo >= z0 && zo < z1)
+            buf_st_f32(ro, lane_b + (uint32_t)(zo - plo) * plane * 4u, newv[T]);
 #pragma unroll
         for (int t = 0; t < T; ++t) sh.v[t][cur][ty + 1][tx + 1] = newv[t];
         __syncthreads();

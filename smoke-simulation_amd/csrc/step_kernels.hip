@@ -8,6 +8,8 @@
 // is one contiguous 256-byte run (FP32) of a 128-byte-aligned row; blockIdx.z
 // walks z planes (kZ planes per thread in the streaming stages).  Compiled
 // with -ffp-contract=off: results are bit-identical to the oracle's FP32 mode.
+#include <algorithm>
+
 #include "step_device.h"
 
 namespace smk {
@@ -709,12 +711,16 @@ hipError_t jacobi_fused(const Grid& g, const float* pin, float* pout, const floa
     if (sweeps == 1)
         return jacobi_sweep(g, pin, pout, d, flags, omega, plain, zero_in, st);
     if (plan.zc < 1) return hipErrorInvalidValue;
+    // The z-chunk is a speed parameter (every chunk gives the same bits): cap
+    // it so a block's span of planes fits the 32-bit buffer offsets.
+    const int zc = std::min(plan.zc, fused_max_zc(g.plane));
+    if (zc < 1) return hipErrorInvalidValue;  // enqueue_jacobi runs single sweeps then
     const bool damped = !plain;
     switch (plan.tile) {
     case JacobiPlan::TILE_32x32:
-        return launch_fused_t<32, 32>(g, pin, pout, d, flags, sweeps, plan.zc, zero_in, omega, damped, st);
+        return launch_fused_t<32, 32>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
     case JacobiPlan::TILE_64x16:
-        return launch_fused_t<64, 16>(g, pin, pout, d, flags, sweeps, plan.zc, zero_in, omega, damped, st);
+        return launch_fused_t<64, 16>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -48,6 +48,39 @@ unsigned long long emu_exchange(unsigned long long bits, int partner);
 int emu_dpp(int old, int src, int ctrl, bool bound_ctrl);
 #define __builtin_amdgcn_update_dpp(old, src, ctrl, row_mask, bank_mask, bound_ctrl) \
     emu_dpp((old), (src), (ctrl), (bound_ctrl))
+
+// Raw buffer access: a base and a size in bytes; loads at or past the size
+// return 0 and such stores are dropped (the hardware's range check for
+// untyped buffers with stride 0).
+struct EmuRsrc {
+    char* base;
+    uint32_t bytes;
+};
+typedef EmuRsrc __amdgpu_buffer_rsrc_t;
+inline EmuRsrc emu_make_rsrc(void* p, int bytes) { return EmuRsrc{static_cast<char*>(p), (uint32_t)bytes}; }
+inline bool emu_in_range(const EmuRsrc& r, int off, int soff, uint32_t size)
+{
+    const uint64_t o = (uint64_t)(uint32_t)off + (uint64_t)(uint32_t)soff;
+    return o + size <= r.bytes;
+}
+inline uint32_t emu_buf_ld32(EmuRsrc r, int off, int soff)
+{
+    uint32_t v = 0;
+    if (emu_in_range(r, off, soff, 4)) std::memcpy(&v, r.base + (uint32_t)off + (uint32_t)soff, 4);
+    return v;
+}
+inline uint8_t emu_buf_ld8(EmuRsrc r, int off, int soff)
+{
+    return emu_in_range(r, off, soff, 1) ? (uint8_t)r.base[(uint32_t)off + (uint32_t)soff] : (uint8_t)0;
+}
+inline void emu_buf_st32(uint32_t v, EmuRsrc r, int off, int soff)
+{
+    if (emu_in_range(r, off, soff, 4)) std::memcpy(r.base + (uint32_t)off + (uint32_t)soff, &v, 4);
+}
+#define __builtin_amdgcn_make_buffer_rsrc(p, stride, bytes, flags) emu_make_rsrc((p), (bytes))
+#define __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, aux) emu_buf_ld32((r), (off), (soff))
+#define __builtin_amdgcn_raw_buffer_load_b8(r, off, soff, aux) emu_buf_ld8((r), (off), (soff))
+#define __builtin_amdgcn_raw_buffer_store_b32(v, r, off, soff, aux) emu_buf_st32((v), (r), (off), (soff))
 template <typename T>
 T __shfl_xor(T v, int mask, int width = 64)
 {
