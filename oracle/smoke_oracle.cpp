@@ -503,18 +503,20 @@ struct Sim : SimBase {
             }
     }
 
-    // Neighbour sum of the pressure form: masked by the neighbour bits, in the
-    // fixed order x-, x+, y-, y+, z-, z+.  (P is +0 on every cell that is not
-    // active, so summing all six neighbours unmasked gives the same bits.)
+    // Neighbour sum of the pressure form at an active (hence interior) cell:
+    // all six neighbours, left to right in the fixed order x-, x+, y-, y+, z-,
+    // z+.  P is exactly +0 on every cell that is not active (never relaxed,
+    // zeroed for warm starts), so this is the sum over the fluid neighbours
+    // s_n P_n of the system acc_i P_i - sum_n s_n P_n = -d_i.
     Real nbr_sum(const std::vector<Real>& A, long ci, uint8_t fl) const
     {
-        Real sum = Real(0);
-        if (fl & F_XM) sum += A[ci - 1];
-        if (fl & F_XP) sum += A[ci + 1];
-        if (fl & F_YM) sum += A[ci - nx];
-        if (fl & F_YP) sum += A[ci + nx];
-        if (fl & F_ZM) sum += A[ci - (long)nx * ny];
-        if (fl & F_ZP) sum += A[ci + (long)nx * ny];
+        (void)fl;
+        const long pl = (long)nx * ny;
+        Real sum = A[ci - 1] + A[ci + 1];
+        sum = sum + A[ci - nx];
+        sum = sum + A[ci + nx];
+        sum = sum + A[ci - pl];
+        sum = sum + A[ci + pl];
         return sum;
     }
 

@@ -203,19 +203,6 @@ __device__ __forceinline__ void divergence_at(const Grid& g, const T* __restrict
     divergence_col<1>(g, u, v, w, flags, d, x, y, z);
 }
 
-__device__ __forceinline__ float nbr_sum(const float* __restrict__ p, int i, uint8_t fl, int px,
-                                         int plane)
-{
-    float sum = 0.0f;
-    if (fl & F_XM) sum += p[i - 1];
-    if (fl & F_XP) sum += p[i + 1];
-    if (fl & F_YM) sum += p[i - px];
-    if (fl & F_YP) sum += p[i + px];
-    if (fl & F_ZM) sum += p[i - plane];
-    if (fl & F_ZP) sum += p[i + plane];
-    return sum;
-}
-
 // RN(1/acc) for acc = 1..6: compile-time constants (correctly rounded by the
 // compiler) picked with selects, so no memory is touched.
 __device__ __forceinline__ float inv_acc(int acc)
@@ -235,15 +222,14 @@ __device__ __forceinline__ float cell_inv(uint8_t fl)
     return (fl & F_ACTIVE) ? inv_acc(__popc(fl & F_NBRS)) : 0.0f;
 }
 
-// The pressure is +0 on every cell that is not active (ring, solid and
-// isolated cells are written 0 and never relaxed), and a running sum that
-// starts at +0 is never -0, so adding all six neighbours unmasked gives the
-// same bits as adding only the fluid ones.
+// Neighbour sum of the pressure forms at an active cell: all six neighbours
+// left to right (x-, x+, y-, y+, z-, z+), as the oracle defines it.  The
+// pressure is exactly +0 on every cell that is not active (ring, solid and
+// isolated cells are written 0 and never relaxed), so this is the sum over the
+// fluid neighbours.
 __device__ __forceinline__ float sum6(float a0, float a1, float a2, float a3, float a4, float a5)
 {
-    float sum = 0.0f;
-    sum = sum + a0;
-    sum = sum + a1;
+    float sum = a0 + a1;
     sum = sum + a2;
     sum = sum + a3;
     sum = sum + a4;
