@@ -528,7 +528,8 @@ struct Sim : SimBase {
     // reference's in-place update equals u* - s_a s_b (P_b - P_a) with P the
     // accumulated per-cell corrections, so the colour sweep is SOR on
     // acc_i P_i - sum_n s_n P_n = -div_i(u*)).  The update of a cell is
-    // G = (sum_n s_n P_n - d_i) * (1/acc_i).
+    // G = (sum_n s_n P_n - d_i) * (1/acc_i), evaluated as one fused
+    // multiply-add: G = fma(sum, inv, 0 - d*inv), inv = RN(1/acc).
     void project_pressure()
     {
         // Right-hand side: d = div(u*) on active cells.
@@ -557,7 +558,8 @@ struct Sim : SimBase {
                             long ci = c(x, y, z);
                             uint8_t fl = flags[ci];
                             if (!(fl & F_ACTIVE)) { B[ci] = Real(0); continue; }
-                            Real G = (nbr_sum(A, ci, fl) - d[ci]) * inv_acc(acc_of(fl));
+                            const Real inv = inv_acc(acc_of(fl));
+                            const Real G = std::fma(nbr_sum(A, ci, fl), inv, Real(0) - d[ci] * inv);
                             B[ci] = plain ? G : A[ci] + om * (G - A[ci]);
                         }
                 P.swap(Q);
@@ -574,7 +576,8 @@ struct Sim : SimBase {
                                 long ci = c(x, y, z);
                                 uint8_t fl = flags[ci];
                                 if (!(fl & F_ACTIVE)) continue;
-                                Real G = (nbr_sum(P, ci, fl) - d[ci]) * inv_acc(acc_of(fl));
+                                const Real inv = inv_acc(acc_of(fl));
+                                const Real G = std::fma(nbr_sum(P, ci, fl), inv, Real(0) - d[ci] * inv);
                                 P[ci] = P[ci] + om * (G - P[ci]);
                             }
                 }

@@ -237,6 +237,15 @@ __device__ __forceinline__ float sum6(float a0, float a1, float a2, float a3, fl
     return sum;
 }
 
+// The pressure forms' update of an active cell, G = (sum - d) * inv, as the
+// oracle defines it: one fused multiply-add fma(sum, inv, 0 - d*inv).  The
+// negated scaled right-hand side 0 - d*inv is a per-cell constant of a solve.
+__device__ __forceinline__ float neg_scaled_rhs(float d, float inv) { return 0.0f - d * inv; }
+__device__ __forceinline__ float pressure_update(float sum, float inv, float d)
+{
+    return __builtin_fmaf(sum, inv, neg_scaled_rhs(d, inv));
+}
+
 // One (damped) Jacobi sweep of acc_i P_i - sum_n s_n P_n = -d_i at an interior
 // cell; inactive cells are written 0.  zero_in: the input is all zeros (cold
 // start), so it is not read.
@@ -254,7 +263,7 @@ __device__ __forceinline__ void jacobi_at(const Grid& g, const float* __restrict
                    pin[i + g.plane]);
         pi = pin[i];
     }
-    float G = (sum - d[i]) * inv;
+    const float G = pressure_update(sum, inv, d[i]);
     float out = plain ? G : pi + omega * (G - pi);
     pout[i] = (inv > 0.0f) ? out : 0.0f;
 }
@@ -385,7 +394,10 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
     __syncthreads();
 
     float w[T][3];  // level t at planes (newest-2, newest-1, newest)
-    float iq[T + 1], dq[T + 1];  // inverse and rhs of planes k, k-1, ..., k-T
+    // Inverse and negated scaled right-hand side 0 - d*inv of planes k, k-1,
+    // ..., k-T.  On cells that are not active inv = 0 and d = 0, so nq = +0
+    // and the plain update fma(sum, 0, +0) is +0 with no select.
+    float iq[T + 1], dq[T + 1];
 #pragma unroll
     for (int t = 0; t < T; ++t) w[t][0] = w[t][1] = w[t][2] = 0.0f;
 #pragma unroll
@@ -426,7 +438,7 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
             dq[t] = dq[t - 1];
         }
         iq[0] = sh.inv[fl];
-        dq[0] = dd;
+        dq[0] = neg_scaled_rhs(dd, iq[0]);
         w[0][0] = w[0][1];
         w[0][1] = w[0][2];
         w[0][2] = p0;
@@ -440,15 +452,15 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
             const float sum = sum6(lane_prev(w[t - 1][1]), lane_next(w[t - 1][1]),
                                    sh.v[t - 1][prev][ty][tx + 1], sh.v[t - 1][prev][ty + 2][tx + 1],
                                    w[t - 1][0], w[t - 1][2]);
-            const float G = (sum - dq[t]) * iq[t];
+            const float G = __builtin_fmaf(sum, iq[t], dq[t]);
             float v;
             if (DAMPED) {
                 const float pc = w[t - 1][1];
                 v = pc + omega * (G - pc);
+                v = (iq[t] > 0.0f) ? v : 0.0f;
             } else {
                 v = G;
             }
-            v = (iq[t] > 0.0f) ? v : 0.0f;
             newv[t] = v;
             if (t < T) {
                 w[t][0] = w[t][1];

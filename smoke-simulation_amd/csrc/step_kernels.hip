@@ -348,9 +348,9 @@ __global__ void k_rbsor_half(Grid g, float* __restrict__ p, const float* __restr
     int i = g.idx(x, y, z);
     uint8_t fl = flags[i];
     if (!(fl & F_ACTIVE)) return;
-    float G = (dev::sum6(p[i - 1], p[i + 1], p[i - g.px], p[i + g.px], p[i - g.plane], p[i + g.plane]) -
-               d[i]) *
-              dev::inv_acc(__popc(fl & F_NBRS));
+    const float G = dev::pressure_update(
+        dev::sum6(p[i - 1], p[i + 1], p[i - g.px], p[i + g.px], p[i - g.plane], p[i + g.plane]),
+        dev::inv_acc(__popc(fl & F_NBRS)), d[i]);
     p[i] = p[i] + omega * (G - p[i]);
 }
 
