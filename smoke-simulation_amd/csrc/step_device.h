@@ -550,6 +550,7 @@ __device__ __forceinline__ float avgW(const T* __restrict__ a, int i, int px, in
 struct Tri {
     float xw0, xw1, yw0, yw1, zw0, zw1;
     int r00, r10, r01, r11;  // idx(x0, y0|y1, z0|z1): first index y, second z
+    int x0, y0, y1, z0, z1;  // the corner coordinates (for windowed reads)
     bool xsame;              // x1 == x0 (position clamped at the last cell)
 };
 
@@ -577,6 +578,11 @@ __device__ __forceinline__ Tri tri_setup(const Grid& g, float px_, float py_, fl
     t.r10 = g.idx(x0, y1, z0);
     t.r01 = g.idx(x0, y0, z1);
     t.r11 = g.idx(x0, y1, z1);
+    t.x0 = x0;
+    t.y0 = y0;
+    t.y1 = y1;
+    t.z0 = z0;
+    t.z1 = z1;
     t.xsame = x1 == x0;
     return t;
 }
@@ -597,12 +603,11 @@ __device__ __forceinline__ float2 ld2(const bf16s* __restrict__ a, int r)
     return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
 }
 
-// The eight corners in the reference's order (:473-481).
-template <typename T>
-__device__ __forceinline__ float tri_eval(const T* __restrict__ a, const Tri& t)
+// The eight corners in the reference's order (:473-481), from the four
+// x-pairs at rows (y0|y1, z0|z1).
+__device__ __forceinline__ float tri_mix(const Tri& t, float2 p00, float2 p10, float2 p01,
+                                         float2 p11)
 {
-    const float2 p00 = ld2(a, t.r00), p10 = ld2(a, t.r10), p01 = ld2(a, t.r01),
-                 p11 = ld2(a, t.r11);
     const float f000 = p00.x, f100 = t.xsame ? p00.x : p00.y;
     const float f010 = p10.x, f110 = t.xsame ? p10.x : p10.y;
     const float f001 = p01.x, f101 = t.xsame ? p01.x : p01.y;
@@ -611,6 +616,11 @@ __device__ __forceinline__ float tri_eval(const T* __restrict__ a, const Tri& t)
            t.xw0 * t.yw1 * t.zw0 * f010 + t.xw1 * t.yw1 * t.zw0 * f110 +
            t.xw0 * t.yw0 * t.zw1 * f001 + t.xw1 * t.yw0 * t.zw1 * f101 +
            t.xw0 * t.yw1 * t.zw1 * f011 + t.xw1 * t.yw1 * t.zw1 * f111;
+}
+template <typename T>
+__device__ __forceinline__ float tri_eval(const T* __restrict__ a, const Tri& t)
+{
+    return tri_mix(t, ld2(a, t.r00), ld2(a, t.r10), ld2(a, t.r01), ld2(a, t.r11));
 }
 
 // velocityAdvectionU/V/W (smokeSimulation.cu:527-615) fused per staggered
@@ -656,6 +666,145 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const T* __res
     if (doU) stf(u1, i, nu);
     if (doV) stf(v1, i, nv);
     if (doW) stf(w1, i, nw);
+}
+
+// LDS window of the tiled advection kernels: NF fields (as float; BF16
+// converts exactly) over the cells [x0-1, x0+AX] x [y0-1, y0+AY] x
+// [z0-1, z0+AZ] of a tile of AX x AY x AZ cells at (x0, y0, z0).
+template <int AX, int AY, int AZ, int NF>
+struct AdvWin {
+    static constexpr int WX = AX + 2, WY = AY + 2, WZ = AZ + 2, N = WX * WY * WZ;
+    float f[NF][N];
+};
+
+// Stages NF fields over the window in x-pairs (one 8-byte FP32 or 4-byte
+// BF16 load per pair; unrolled: every load of a thread is in flight at
+// once).  wx0 = x0 - 1 is even (tiles start at x0 = 1 + AX*m, AX even) and
+// WX is even, so a pair starts at an even x and never straddles a row; rows
+// hold px >= nx + 2 elements when nx is even (px is a multiple of 32), so the
+// pair of a window cell x <= nx lies inside its row.  Its second element may
+// be past nx: that cell is never read.  Pairs past (nx, ny, nz) are left at
+// zero; the rest lie inside the allocation.
+template <int AX, int AY, int AZ, int NF, typename T>
+__device__ __forceinline__ void stage_window(const Grid& g, const T* const (&src)[NF], int wx0,
+                                             int wy0, int wz0, AdvWin<AX, AY, AZ, NF>& win)
+{
+    using W = AdvWin<AX, AY, AZ, NF>;
+    static_assert(AX % 2 == 0 && W::WX % 2 == 0, "x-pairs start at even x within a row");
+    constexpr int kPairs = W::N / 2;
+    const int tid = (int)(threadIdx.x + BX * threadIdx.y);
+    constexpr int kLoads = (kPairs + BX * BY - 1) / (BX * BY);
+#pragma unroll
+    for (int it = 0; it < kLoads; ++it) {
+        const int k2 = tid + it * (BX * BY);
+        if (k2 >= kPairs) break;
+        const int k = 2 * k2;
+        const int lx = k % W::WX, r = k / W::WX, ly = r % W::WY, lz = r / W::WY;
+        const int xw = wx0 + lx, yw = wy0 + ly, zw = wz0 + lz;
+        float2 a[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) a[f] = make_float2(0.0f, 0.0f);
+        if (xw <= g.nx && yw <= g.ny && zw <= g.nz) {
+            const int i = g.idx(xw, yw, zw);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) a[f] = ld2(src[f], i);
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            win.f[f][k] = a[f].x;
+            win.f[f][k + 1] = a[f].y;
+        }
+    }
+}
+
+// One sample from the window when its four x-pairs lie inside it, else from
+// global memory: the same eight values either way.
+template <int WX, int WY, int WZ, typename T>
+__device__ __forceinline__ float tri_win(const T* __restrict__ a, const float* s, const Tri& t,
+                                         int wx0, int wy0, int wz0)
+{
+    const int lx = t.x0 - wx0, ly0 = t.y0 - wy0, ly1 = t.y1 - wy0, lz0 = t.z0 - wz0,
+              lz1 = t.z1 - wz0;
+    if (lx >= 0 && lx <= WX - 2 && ly0 >= 0 && ly1 <= WY - 1 && lz0 >= 0 && lz1 <= WZ - 1) {
+        const int l00 = lx + WX * (ly0 + WY * lz0), l10 = lx + WX * (ly1 + WY * lz0),
+                  l01 = lx + WX * (ly0 + WY * lz1), l11 = lx + WX * (ly1 + WY * lz1);
+        return tri_mix(t, make_float2(s[l00], s[l00 + 1]), make_float2(s[l10], s[l10 + 1]),
+                       make_float2(s[l01], s[l01 + 1]), make_float2(s[l11], s[l11 + 1]));
+    }
+    return tri_eval(a, t);
+}
+
+// advect_velocity_at over one tile of AX x AY x AZ cells at (x0, y0, z0) with
+// a block of BX x BY threads (AX == BX): the block stages the window, then
+// each thread takes the cells (x0 + threadIdx.x, y0 + jy, z0 + jz) for
+// jy = threadIdx.y, threadIdx.y + BY, ... and reads the own components and
+// the averages (all inside the window: faces x+1, y+1, x-1, y-1, z-1) and
+// every sample whose corners fall inside it from LDS.  A back-trace longer
+// than one cell leaves the window and reads global memory.  Window cells past
+// (nx, ny, nz) are never read (averages reach x <= nx, y <= ny; clamped
+// corners stay below n) and are left at zero; every window cell is inside
+// the allocation otherwise.  Same values, same arithmetic as
+// advect_velocity_at: the results are identical.
+template <int AX, int AY, int AZ, typename T>
+__device__ __forceinline__ void advect_velocity_tile(const Grid& g, const T* __restrict__ u0,
+                                                     const T* __restrict__ v0,
+                                                     const T* __restrict__ w0,
+                                                     T* __restrict__ u1, T* __restrict__ v1,
+                                                     T* __restrict__ w1,
+                                                     const uint8_t* __restrict__ flags, float dt,
+                                                     int x0, int y0, int z0,
+                                                     AdvWin<AX, AY, AZ, 3>& win)
+{
+    static_assert(AX == BX && AY % BY == 0, "a row of the tile per wave, whole rows per thread");
+    constexpr int kRows = AY / BY;
+    static_assert(AZ * kRows <= 8, "the flags of a thread's cells fit one 64-bit word");
+    using W = AdvWin<AX, AY, AZ, 3>;
+    const int wx0 = x0 - 1, wy0 = y0 - 1, wz0 = z0 - 1;
+    const int x = x0 + (int)threadIdx.x;
+    // The flags of the thread's cells (cell c = jz * kRows + jr in byte c),
+    // loaded with the window; a cell outside [1, n-1] gets 0 (no work).
+    unsigned long long fls = 0;
+#pragma unroll
+    for (int jz = 0; jz < AZ; ++jz)
+#pragma unroll
+        for (int jr = 0; jr < kRows; ++jr) {
+            const int y = y0 + (int)threadIdx.y + jr * BY, z = z0 + jz;
+            if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz - 1)
+                fls |= (unsigned long long)flags[g.idx(x, y, z)] << (8 * (jz * kRows + jr));
+        }
+    const T* const src[3] = {u0, v0, w0};
+    stage_window(g, src, wx0, wy0, wz0, win);
+    __syncthreads();
+    if (fls == 0) return;
+    const float xf = (float)x;
+    for (int c = 0; c < AZ * kRows; ++c) {
+        const uint8_t fl = (uint8_t)(fls >> (8 * c));
+        const int y = y0 + (int)threadIdx.y + (c % kRows) * BY, z = z0 + c / kRows;
+        const bool self = (fl & F_SELF) != 0;
+        const bool doU = self && y < g.ny - 1 && z < g.nz - 1 && (fl & F_XM);
+        const bool doV = self && x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM);
+        const bool doW = self && x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM);
+        if (!(doU || doV || doW)) continue;
+        const int i = g.idx(x, y, z);
+        const int l = (x - wx0) + W::WX * ((y - wy0) + W::WY * (z - wz0));
+        const float uu = win.f[0][l], vv = win.f[1][l], ww = win.f[2][l];
+        const float aU = avgU(win.f[0], l, W::WX, W::WX * W::WY);
+        const float aV = avgV(win.f[1], l, W::WX, W::WX * W::WY);
+        const float aW = avgW(win.f[2], l, W::WX, W::WX * W::WY);
+        const float yf = (float)y, zf = (float)z;
+        const Tri tu = tri_setup(g, xf - dt * uu, (yf + 0.5f) - dt * aV,
+                                 (zf + 0.5f) - dt * aW, 0.0f, 0.5f, 0.5f);
+        const Tri tv = tri_setup(g, (xf + 0.5f) - dt * aU, yf - dt * vv,
+                                 (zf + 0.5f) - dt * aW, 0.5f, 0.0f, 0.5f);
+        const Tri tw = tri_setup(g, (xf + 0.5f) - dt * aU, (yf + 0.5f) - dt * aV,
+                                 zf - dt * ww, 0.5f, 0.5f, 0.0f);
+        const float nu = tri_win<W::WX, W::WY, W::WZ>(u0, win.f[0], tu, wx0, wy0, wz0);
+        const float nv = tri_win<W::WX, W::WY, W::WZ>(v0, win.f[1], tv, wx0, wy0, wz0);
+        const float nw = tri_win<W::WX, W::WY, W::WZ>(w0, win.f[2], tw, wx0, wy0, wz0);
+        if (doU) stf(u1, i, nu);
+        if (doV) stf(v1, i, nv);
+        if (doW) stf(w1, i, nw);
+    }
 }
 
 // advectSmoke (smokeSimulation.cu:617-638): cell velocity from the

@@ -314,12 +314,21 @@ __global__ void k_jacobi(Grid g, const float* __restrict__ pin, float* __restric
                    blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
 }
 
+// Tile order of the tiled kernels.  Workgroup b runs on XCD b % 8 (8 XCDs,
+// each with its own L2): the tiles are renumbered so that each XCD gets one
+// contiguous run of tiles (x fastest), and tiles that share halo cells run on
+// the same L2 at the same time.  The renumbering is a permutation (the tail
+// beyond a multiple of 8 keeps its index).
+__device__ __forceinline__ int xcd_tile_order()
+{
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int nb = nbx * nby * (int)gridDim.z;
+    const int b = (int)blockIdx.x + nbx * ((int)blockIdx.y + nby * (int)blockIdx.z);
+    const int full = nb & ~7;
+    return (b < full) ? (b & 7) * (full >> 3) + (b >> 3) : b;
+}
+
 // T fused Jacobi sweeps, one tile per block (dev::jacobi_fused_tile).
-// Workgroup b runs on XCD b % 8 (8 XCDs, each with its own L2): the tiles are
-// renumbered so that each XCD gets one contiguous run of tiles (x fastest),
-// and tiles that share halo columns run on the same L2 at the same time.
-// The renumbering is a permutation (the tail beyond a multiple of 8 keeps its
-// index).
 template <int TX, int TY, int T, bool DAMPED>
 __global__ __launch_bounds__(TX * TY) void k_jacobi_fused(
     Grid g, const float* __restrict__ pin, float* __restrict__ pout, const float* __restrict__ d,
@@ -327,10 +336,7 @@ __global__ __launch_bounds__(TX * TY) void k_jacobi_fused(
 {
     __shared__ dev::FusedLds<TX, TY, T> sh;
     const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
-    const int nb = nbx * nby * (int)gridDim.z;
-    const int b = (int)blockIdx.x + nbx * ((int)blockIdx.y + nby * (int)blockIdx.z);
-    const int full = nb & ~7;
-    const int L = (b < full) ? (b & 7) * (full >> 3) + (b >> 3) : b;
+    const int L = xcd_tile_order();
     dev::jacobi_fused_tile<TX, TY, T, DAMPED>(g, pin, pout, d, flags, zc, zero_in, omega, L % nbx,
                                               (L / nbx) % nby, L / (nbx * nby), sh);
 }
@@ -373,13 +379,22 @@ __global__ void k_gradient(Grid g, T* __restrict__ u, T* __restrict__ v, T* __re
                           blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
+// Velocity advection tile: kAdvX x kAdvY x kAdvZ cells per block of BX x BY
+// threads (dev::advect_velocity_tile), in XCD tile order.
+constexpr int kAdvX = BX, kAdvY = 4, kAdvZ = 2;
+
 template <typename T>
-__global__ void k_advect_velocity(Grid g, const T* __restrict__ u0, const T* __restrict__ v0,
-                                  const T* __restrict__ w0, T* __restrict__ u1, T* __restrict__ v1,
-                                  T* __restrict__ w1, const uint8_t* __restrict__ flags, float dt)
+__global__ __launch_bounds__(BX * BY) void k_advect_velocity(
+    Grid g, const T* __restrict__ u0, const T* __restrict__ v0, const T* __restrict__ w0,
+    T* __restrict__ u1, T* __restrict__ v1, T* __restrict__ w1, const uint8_t* __restrict__ flags,
+    float dt)
 {
-    dev::advect_velocity_at(g, u0, v0, w0, u1, v1, w1, flags, dt, blockIdx.x * BX + threadIdx.x + 1,
-                            blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
+    __shared__ dev::AdvWin<kAdvX, kAdvY, kAdvZ, 3> win;
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int L = xcd_tile_order();
+    dev::advect_velocity_tile<kAdvX, kAdvY, kAdvZ>(
+        g, u0, v0, w0, u1, v1, w1, flags, dt, 1 + kAdvX * (L % nbx), 1 + kAdvY * ((L / nbx) % nby),
+        1 + kAdvZ * (L / (nbx * nby)), win);
 }
 
 template <typename T>
@@ -767,8 +782,11 @@ static hipError_t launch_advect_velocity(const Grid& g, const T* u0, const T* v0
                                          T* u1, T* v1, T* w1, const uint8_t* flags, float dt,
                                          hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advect_velocity<T>, grid_for(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY),
-                       0, st, g, u0, v0, w0, u1, v1, w1, flags, dt);
+    const dim3 grid((unsigned)((g.nx - 1 + kAdvX - 1) / kAdvX),
+                    (unsigned)((g.ny - 1 + kAdvY - 1) / kAdvY),
+                    (unsigned)((g.nz - 1 + kAdvZ - 1) / kAdvZ));
+    hipLaunchKernelGGL(k_advect_velocity<T>, grid, dim3(BX, BY), 0, st, g, u0, v0, w0, u1, v1, w1,
+                       flags, dt);
     return hipGetLastError();
 }
 hipError_t advect_velocity(const Grid& g, const float* u0, const float* v0, const float* w0,

@@ -212,9 +212,25 @@ def case_vorticity():
         g.close()
 
 
+def case_advection():
+    # Velocity advection tiles: grids wider than one 64-cell tile row with
+    # partial tiles, and back-traces longer than the tile halo (large noise
+    # and dt) next to short ones.
+    for dims, seed, noise, dt in [((70, 11, 10), 40, 0.5, 0.05), ((67, 18, 9), 41, 40.0, 0.1),
+                                  ((130, 9, 6), 42, 8.0, 0.3)]:
+        sc = scenes.test_scene(dims, seed=seed, projection=2, iterations=4, noise=noise)
+        sc["dt"] = dt
+        g, o = gpu_for(sc), oracle_for(sc, 32)
+        for _ in range(2):
+            g.step(sc["dt"], readback=False)
+        o.step(sc["dt"], 2)
+        assert_bitwise_equal(g.state(), o.state(), f"advection {dims}")
+        g.close()
+
+
 CASES = {"single": case_single, "plans": case_plans, "batch": case_batch,
          "server": case_server, "readback": case_readback, "precision": case_precision,
-         "vorticity": case_vorticity}
+         "vorticity": case_vorticity, "advection": case_advection}
 
 
 def main():

@@ -114,6 +114,24 @@ def test_moving_objects_and_parameter_changes():
         assert_bitwise_equal(g.state(), o.state(), f"step {k}")
 
 
+@pytest.mark.parametrize("dims,seed,noise,dt", [((70, 11, 10), 40, 0.5, 0.05),
+                                                 ((67, 18, 9), 41, 40.0, 0.1),
+                                                 ((130, 9, 6), 42, 8.0, 0.3),
+                                                 ((129, 66, 35), 43, 6.0, 0.25)])
+def test_velocity_advection_tiles_bitwise(dims, seed, noise, dt):
+    # Velocity advection stages 64 x 4 x 2 cell tiles with a one-cell halo:
+    # grids wider than one tile with partial tiles on every axis, and
+    # back-traces longer than one cell (the last two cases: up to 2.7 cells
+    # per step at their noise and dt) that read outside the tile.
+    sc = scenes.test_scene(dims, seed=seed, projection=JAC, iterations=4, noise=noise)
+    sc["dt"] = dt
+    g, o = gpu_for(sc), oracle_for(sc, 32)
+    for k in range(2):
+        g.step(dt, readback=False)
+        o.step(dt)
+        assert_bitwise_equal(g.state(), o.state(), f"step {k}")
+
+
 def test_max_objects_and_errors():
     g = smokesim.Simulation(8, 8, 8)
     for i in range(smokesim.MAX_OBJECTS):
