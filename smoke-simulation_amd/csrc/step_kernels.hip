@@ -4,10 +4,13 @@
 //
 // The arithmetic lives in step_device.h (shared with the batched kernels);
 // the kernels here map threads to cells.  Thread blocks are 64 x 4 x 1: one
-// wave64 covers 64 consecutive x of one row, so every field access of a wave
-// is one contiguous 256-byte run (FP32) of a 128-byte-aligned row; blockIdx.z
-// walks z planes (kZ planes per thread in the streaming stages).  Compiled
-// with -ffp-contract=off: results are bit-identical to the oracle's FP32 mode.
+// wave64 covers 64 consecutive x of one row starting at a multiple of 64 (a
+// stage that starts at x = 1 idles lane 0 of a row's first wave), so a
+// wave's own-cell accesses are one contiguous 256-byte run (FP32) on two
+// 128-byte lines of a 128-byte-aligned row, shared with no other wave;
+// blockIdx.z walks z planes (kZ planes per thread in the streaming stages).
+// Compiled with -ffp-contract=off: results are bit-identical to the oracle's
+// FP32 mode.
 #include <algorithm>
 
 #include "step_device.h"
@@ -302,7 +305,7 @@ __global__ void k_divergence(Grid g, const T* __restrict__ u, const T* __restric
                              const T* __restrict__ w, const uint8_t* __restrict__ flags,
                              float* __restrict__ d)
 {
-    dev::divergence_col<kZ>(g, u, v, w, flags, d, blockIdx.x * BX + threadIdx.x + 1,
+    dev::divergence_col<kZ>(g, u, v, w, flags, d, blockIdx.x * BX + threadIdx.x,
                             blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
@@ -310,7 +313,7 @@ __global__ void k_jacobi(Grid g, const float* __restrict__ pin, float* __restric
                          const float* __restrict__ d, const uint8_t* __restrict__ flags,
                          float omega, int plain, int zero_in)
 {
-    dev::jacobi_at(g, pin, pout, d, flags, omega, plain, zero_in, blockIdx.x * BX + threadIdx.x + 1,
+    dev::jacobi_at(g, pin, pout, d, flags, omega, plain, zero_in, blockIdx.x * BX + threadIdx.x,
                    blockIdx.y * BY + threadIdx.y + 1, blockIdx.z + 1);
 }
 
@@ -375,7 +378,7 @@ template <typename T>
 __global__ void k_gradient(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
                            const float* __restrict__ p, const uint8_t* __restrict__ flags)
 {
-    dev::gradient_col<kZ>(g, u, v, w, p, flags, blockIdx.x * BX + threadIdx.x + 1,
+    dev::gradient_col<kZ>(g, u, v, w, p, flags, blockIdx.x * BX + threadIdx.x,
                           blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
@@ -402,7 +405,7 @@ __global__ void k_advect_density(Grid g, const T* __restrict__ r0, T* __restrict
                                  const T* __restrict__ u, const T* __restrict__ v,
                                  const T* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
 {
-    dev::advect_density_col<kZ>(g, r0, r1, u, v, w, flags, dt, blockIdx.x * BX + threadIdx.x + 1,
+    dev::advect_density_col<kZ>(g, r0, r1, u, v, w, flags, dt, blockIdx.x * BX + threadIdx.x,
                                 blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
@@ -665,7 +668,7 @@ template <typename T>
 static hipError_t launch_divergence(const Grid& g, const T* u, const T* v, const T* w,
                                     const uint8_t* flags, float* d, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_divergence<T>, grid_col(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st,
+    hipLaunchKernelGGL(k_divergence<T>, grid_col(g.nx - 1, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st,
                        g, u, v, w, flags, d);
     return hipGetLastError();
 }
@@ -683,7 +686,7 @@ hipError_t divergence(const Grid& g, const bf16s* u, const bf16s* v, const bf16s
 hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const float* d,
                         const uint8_t* flags, float omega, int plain, int zero_in, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_jacobi, grid_for(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
+    hipLaunchKernelGGL(k_jacobi, grid_for(g.nx - 1, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
                        pin, pout, d, flags, omega, plain, zero_in);
     return hipGetLastError();
 }
@@ -762,7 +765,7 @@ template <typename T>
 static hipError_t launch_gradient(const Grid& g, T* u, T* v, T* w, const float* p,
                                   const uint8_t* flags, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gradient<T>, grid_col(g.nx - 1, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st,
+    hipLaunchKernelGGL(k_gradient<T>, grid_col(g.nx, g.ny - 1, g.nz - 1), dim3(BX, BY), 0, st,
                        g, u, v, w, p, flags);
     return hipGetLastError();
 }
@@ -806,7 +809,7 @@ template <typename T>
 static hipError_t launch_advect_density(const Grid& g, const T* r0, T* r1, const T* u, const T* v,
                                         const T* w, const uint8_t* flags, float dt, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advect_density<T>, grid_col(g.nx - 2, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
+    hipLaunchKernelGGL(k_advect_density<T>, grid_col(g.nx - 1, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
                        st, g, r0, r1, u, v, w, flags, dt);
     return hipGetLastError();
 }
