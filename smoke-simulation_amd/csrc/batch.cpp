@@ -89,8 +89,11 @@ void fill_table(smk::StageTable& t, const std::vector<smoke_sim*>& sims, int kin
             gy = ceil_div(g.ny - 2, ty - 2 * T);
             gz = ceil_div(g.nz - 2, plan.zc);
         } else {
+            // Cells from x = 0 (the rows' line-aligned start; a stage starting
+            // at x = 1 idles lane 0), y and z from 1 except for the force stage.
             const int sub = kind == kTabForce ? 0 : (kind == kTabFaces ? 1 : 2);
-            const int wx = g.nx - sub, wy = g.ny - (kind == kTabForce ? 1 : sub), wz = g.nz - sub;
+            const int wx = g.nx - sub + (kind == kTabForce ? 0 : 1);
+            const int wy = g.ny - (kind == kTabForce ? 1 : sub), wz = g.nz - sub;
             gx = ceil_div(wx, smk::kBlockX);
             gy = ceil_div(wy, smk::kBlockY);
             gz = wz;

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
         const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
         const SimDesc& s = sims[si];
         const int n = s.now, p = 1 - s.now;
-        const int x = bx * BX + (int)threadIdx.x + (STAGE == BS_FORCE ? 0 : 1);
+        const int x = bx * BX + (int)threadIdx.x;
         const int y = by * BY + (int)threadIdx.y + 1;
         const int z = bz + (STAGE == BS_FORCE ? 0 : 1);
         if (STAGE == BS_FORCE)
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(BX * BY) void kb_jacobi1(const SimDesc* __restrict_
         const SimDesc& s = sims[si];
         const int in = s.pcur ^ lp;
         dev::jacobi_at(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, omega, plain, zero_in,
-                       bx * BX + (int)threadIdx.x + 1, by * BY + (int)threadIdx.y + 1, bz + 1);
+                       bx * BX + (int)threadIdx.x, by * BY + (int)threadIdx.y + 1, bz + 1);
     }
 }
 
