@@ -93,12 +93,33 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
             dev::divergence_at(s.g, s.u[n], s.v[n], s.w[n], s.flags, s.d, x, y, z);
         else if (STAGE == BS_GRAD)
             dev::gradient_at(s.g, s.u[n], s.v[n], s.w[n], s.P[s.pcur ^ lp], s.flags, x, y, z);
-        else if (STAGE == BS_ADV_V)
-            dev::advect_velocity_at(s.g, s.u[n], s.v[n], s.w[n], s.u[p], s.v[p], s.w[p], s.flags,
-                                    s.force.dt, x, y, z);
         else
             dev::advect_density_at(s.g, s.rho[n], s.rho[p], s.u[p], s.v[p], s.w[p], s.flags,
                                    s.force.dt, x, y, z);
+    }
+}
+
+// Velocity advection over every tile of every simulation
+// (dev::advect_velocity_tile); the barrier at the end of a tile keeps the
+// next tile's staging from overwriting a window still being read.  Every
+// thread of a workgroup walks the same tiles.
+__global__ __launch_bounds__(BX * BY) void kb_adv_v(const SimDesc* __restrict__ sims,
+                                                    const StageTable* __restrict__ tab, int nsims)
+{
+    __shared__ dev::AdvWin<BX, kAdvTileY, kAdvTileZ, 3> win;
+    const int total = tab->start[nsims];
+    const VbRange r = vb_range(total);
+    for (int vb = r.begin; vb < r.end; vb += r.stride) {
+        const int si = find_owner(tab->start, nsims, vb);
+        const int lb = vb - tab->start[si];
+        const int gx = tab->gx[si], gy = tab->gy[si];
+        const SimDesc& s = sims[si];
+        const int n = s.now, p = 1 - s.now;
+        dev::advect_velocity_tile<BX, kAdvTileY, kAdvTileZ>(
+            s.g, s.u[n], s.v[n], s.w[n], s.u[p], s.v[p], s.w[p], s.flags, s.force.dt,
+            1 + BX * (lb % gx), 1 + kAdvTileY * ((lb / gx) % gy), 1 + kAdvTileZ * (lb / (gx * gy)),
+            win);
+        __syncthreads();
     }
 }
 
@@ -206,7 +227,6 @@ hipError_t batch_cells(int stage, const SimDesc* sims, const StageTable* tab, in
     case BS_FORCE: hipLaunchKernelGGL(kb_cells<BS_FORCE>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     case BS_DIV: hipLaunchKernelGGL(kb_cells<BS_DIV>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     case BS_GRAD: hipLaunchKernelGGL(kb_cells<BS_GRAD>, gd, bd, 0, st, sims, tab, nsims, lp); break;
-    case BS_ADV_V: hipLaunchKernelGGL(kb_cells<BS_ADV_V>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     case BS_ADV_D: hipLaunchKernelGGL(kb_cells<BS_ADV_D>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     default: return hipErrorInvalidValue;
     }
@@ -242,7 +262,17 @@ hipError_t batch_flip(SimDesc* sims, int nsims, int pflip, hipStream_t st)
     return hipGetLastError();
 }
 
-int batch_cells_blocks_per_cu() { return occupancy(kb_cells<BS_ADV_V>, BX * BY); }
+int batch_cells_blocks_per_cu() { return occupancy(kb_cells<BS_ADV_D>, BX * BY); }
+
+hipError_t batch_advect_velocity(const SimDesc* sims, const StageTable* tab, int nsims,
+                                 int total_blocks, int grid, hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_adv_v, dim3((unsigned)grid), dim3(BX, BY), 0, st, sims, tab, nsims);
+    return hipGetLastError();
+}
+
+int batch_advect_velocity_blocks_per_cu() { return occupancy(kb_adv_v, BX * BY); }
 
 int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps)
 {

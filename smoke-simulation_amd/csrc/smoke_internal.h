@@ -116,6 +116,11 @@ struct SimDesc {
     int pcur;         // P buffer holding the latest pressure
 };
 
+// Cells per tile of the velocity advection (x spans one block row of
+// kBlockX cells): kAdvTileY rows x kAdvTileZ planes, staged in LDS with a
+// one-cell halo.
+constexpr int kAdvTileY = 4, kAdvTileZ = 2;
+
 // Work list of one batched stage: simulation s owns virtual blocks
 // [start[s], start[s+1]) arranged as a gx[s] x gy[s] x (rest) block grid.
 constexpr int kMaxBatch = 256;
@@ -152,6 +157,11 @@ hipError_t batch_jacobi(const SimDesc* sims, const StageTable* tab, int nsims, i
 hipError_t batch_flip(SimDesc* sims, int nsims, int pflip, hipStream_t st);
 // Resident blocks per CU of the batched kernels (for sizing persistent grids).
 int batch_cells_blocks_per_cu();
+// Velocity advection of every simulation in tiles (kBlockX x kAdvTileY x
+// kAdvTileZ cells over [1, n-1]^3, table built with those tiles).
+hipError_t batch_advect_velocity(const SimDesc* sims, const StageTable* tab, int nsims,
+                                 int total_blocks, int grid, hipStream_t st);
+int batch_advect_velocity_blocks_per_cu();
 int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps);
 
 // ------------------------------------------------------------ launchers

@@ -384,7 +384,7 @@ __global__ void k_gradient(Grid g, T* __restrict__ u, T* __restrict__ v, T* __re
 
 // Velocity advection tile: kAdvX x kAdvY x kAdvZ cells per block of BX x BY
 // threads (dev::advect_velocity_tile), in XCD tile order.
-constexpr int kAdvX = BX, kAdvY = 4, kAdvZ = 2;
+constexpr int kAdvX = BX, kAdvY = kAdvTileY, kAdvZ = kAdvTileZ;
 
 template <typename T>
 __global__ __launch_bounds__(BX * BY) void k_advect_velocity(
