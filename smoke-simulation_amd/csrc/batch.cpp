@@ -103,7 +103,7 @@ void fill_table(smk::StageTable& t, const std::vector<smoke_sim*>& sims, int kin
             const int wy = g.ny - (kind == kTabForce ? 1 : sub), wz = g.nz - sub;
             gx = ceil_div(wx, smk::kBlockX);
             gy = ceil_div(wy, smk::kBlockY);
-            gz = wz;
+            gz = ceil_div(wz, smk::kColZ);
         }
         t.start[s] = acc;
         t.gx[s] = gx;

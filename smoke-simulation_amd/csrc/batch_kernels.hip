@@ -86,16 +86,18 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
         const int n = s.now, p = 1 - s.now;
         const int x = bx * BX + (int)threadIdx.x;
         const int y = by * BY + (int)threadIdx.y + 1;
-        const int z = bz + (STAGE == BS_FORCE ? 0 : 1);
+        const int z0 = bz * kColZ + (STAGE == BS_FORCE ? 0 : 1);
         if (STAGE == BS_FORCE)
-            dev::force_at(s.g, s.u[n], s.v[n], s.w[n], s.rho[n], s.flags, s.force, x, y, z);
+            dev::force_col<kColZ>(s.g, s.u[n], s.v[n], s.w[n], s.rho[n], s.flags, s.force, x, y,
+                                  z0);
         else if (STAGE == BS_DIV)
-            dev::divergence_at(s.g, s.u[n], s.v[n], s.w[n], s.flags, s.d, x, y, z);
+            dev::divergence_col<kColZ>(s.g, s.u[n], s.v[n], s.w[n], s.flags, s.d, x, y, z0);
         else if (STAGE == BS_GRAD)
-            dev::gradient_at(s.g, s.u[n], s.v[n], s.w[n], s.P[s.pcur ^ lp], s.flags, x, y, z);
+            dev::gradient_col<kColZ>(s.g, s.u[n], s.v[n], s.w[n], s.P[s.pcur ^ lp], s.flags, x, y,
+                                     z0);
         else
-            dev::advect_density_at(s.g, s.rho[n], s.rho[p], s.u[p], s.v[p], s.w[p], s.flags,
-                                   s.force.dt, x, y, z);
+            dev::advect_density_col<kColZ>(s.g, s.rho[n], s.rho[p], s.u[p], s.v[p], s.w[p],
+                                           s.flags, s.force.dt, x, y, z0);
     }
 }
 
@@ -159,8 +161,12 @@ __global__ __launch_bounds__(BX * BY) void kb_jacobi1(const SimDesc* __restrict_
         const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
         const SimDesc& s = sims[si];
         const int in = s.pcur ^ lp;
-        dev::jacobi_at(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, omega, plain, zero_in,
-                       bx * BX + (int)threadIdx.x, by * BY + (int)threadIdx.y + 1, bz + 1);
+        // The interior table's blocks hold kColZ planes each.
+#pragma unroll
+        for (int j = 0; j < kColZ; ++j)
+            dev::jacobi_at(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, omega, plain, zero_in,
+                           bx * BX + (int)threadIdx.x, by * BY + (int)threadIdx.y + 1,
+                           bz * kColZ + j + 1);
     }
 }
 

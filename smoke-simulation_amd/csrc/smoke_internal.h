@@ -116,6 +116,10 @@ struct SimDesc {
     int pcur;         // P buffer holding the latest pressure
 };
 
+// Consecutive z planes per thread in the per-cell streaming stages (force,
+// divergence, gradient, density advection), single and batched.
+constexpr int kColZ = 4;
+
 // Cells per tile of the velocity advection (x spans one block row of
 // kBlockX cells): kAdvTileY rows x kAdvTileZ planes, staged in LDS with a
 // one-cell halo.

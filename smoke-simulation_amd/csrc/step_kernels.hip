@@ -30,7 +30,7 @@ inline dim3 grid_for(int wx, int wy, int wz)
 // Cells per thread of the streaming stages (force, divergence, gradient,
 // density advection): kZ consecutive planes of one column, so each thread
 // has kZ cells' loads in flight (the *_col bodies of step_device.h).
-constexpr int kZ = 4;
+constexpr int kZ = kColZ;
 inline dim3 grid_col(int wx, int wy, int wz)
 {
     return dim3((unsigned)((wx + BX - 1) / BX), (unsigned)((wy + BY - 1) / BY),
