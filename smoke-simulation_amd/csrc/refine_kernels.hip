@@ -95,12 +95,12 @@ __global__ __launch_bounds__(BX * BY) void k_ref_norm0(Grid g, const float* __re
                                                        double* __restrict__ part,
                                                        unsigned* __restrict__ pmax)
 {
-    const int x = blockIdx.x * BX + threadIdx.x + 1;
+    const int x = blockIdx.x * BX + threadIdx.x;  // lane x = cell x: lane 0 idles
     const int y = blockIdx.y * BY + threadIdx.y + 1;
     const int z0 = blockIdx.z * kZB + 1;
     double s2 = 0.0;
     float m = 0.0f;
-    if (x <= g.nx - 2 && y <= g.ny - 2) {
+    if (x >= 1 && x <= g.nx - 2 && y <= g.ny - 2) {
         float r[kZB];
 #pragma unroll
         for (int k = 0; k < kZB; ++k) {
@@ -142,13 +142,13 @@ __global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __re
 {
     const int done = ctl->done;
     const float sinv = ctl->sinv;
-    const int x = blockIdx.x * BX + threadIdx.x + 1;
+    const int x = blockIdx.x * BX + threadIdx.x;  // lane x = cell x: lane 0 idles
     const int y = blockIdx.y * BY + threadIdx.y + 1;
     const int z0 = blockIdx.z * kZB + 1;
     const int nk = min(kZB, g.nz - 1 - z0);  // planes of this block (>= 1)
     double s2 = 0.0;
     float m = 0.0f;
-    if (x <= g.nx - 2 && y <= g.ny - 2) {
+    if (x >= 1 && x <= g.nx - 2 && y <= g.ny - 2) {
         const int i0 = g.idx(x, y, z0);
         if (done) {
 #pragma unroll
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
 
 inline dim3 resid_grid(const Grid& g)
 {
-    return dim3((unsigned)((g.nx - 2 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
+    return dim3((unsigned)((g.nx - 1 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
                 (unsigned)((g.nz - 2 + kZB - 1) / kZB));
 }
 
