@@ -383,7 +383,8 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
     const uint32_t plane = (uint32_t)g.plane;
     const uint32_t span = (uint32_t)(phi - plo) * plane;  // elements
     const size_t first = (size_t)plo * (size_t)g.plane;
-    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(pin + first, span * 4u);
+    // zero_in: P reads as 0 everywhere (an empty descriptor), with no branch.
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(pin + first, zero_in ? 0u : span * 4u);
     const __amdgpu_buffer_rsrc_t rd = buf_rsrc(d + first, span * 4u);
     const __amdgpu_buffer_rsrc_t rf = buf_rsrc(flags + first, span);
     const __amdgpu_buffer_rsrc_t ro = buf_rsrc(pout + first, span * 4u);
@@ -409,7 +410,7 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
     uint8_t nf[kPrefetch];
     auto load = [&](int kn, float& p, float& dv, uint8_t& f) {
         const uint32_t dk = (uint32_t)(kn - plo) * plane;
-        p = zero_in ? 0.0f : buf_f32(rp, lane_b + dk * 4u);
+        p = buf_f32(rp, lane_b + dk * 4u);
         dv = buf_f32(rd, lane_b + dk * 4u);
         f = buf_u8(rf, lane_e + dk);
     };
