@@ -93,8 +93,8 @@ void fill_table(smk::StageTable& t, const std::vector<smoke_sim*>& sims, int kin
             gz = ceil_div(g.nz - 2, plan.zc);
         } else if (kind == kTabAdvV) {
             gx = ceil_div(g.nx - 1, smk::kBlockX);
-            gy = ceil_div(g.ny - 1, smk::kAdvTileY);
-            gz = ceil_div(g.nz - 1, smk::kAdvTileZ);
+            gy = ceil_div(g.ny - 1, smk::kBlockY);
+            gz = ceil_div(g.nz - 1, smk::kAdvChunkZ);
         } else {
             // Cells from x = 0 (the rows' line-aligned start; a stage starting
             // at x = 1 idles lane 0), y and z from 1 except for the force stage.

@@ -101,14 +101,15 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
     }
 }
 
-// Velocity advection over every tile of every simulation
-// (dev::advect_velocity_tile); the barrier at the end of a tile keeps the
-// next tile's staging from overwriting a window still being read.  Every
-// thread of a workgroup walks the same tiles.
+// Velocity advection over every column of every simulation
+// (dev::advect_velocity_march: BX x BY cells marched through kAdvChunkZ
+// planes); the barrier after a column keeps the next column's ring stores
+// from overwriting planes still being read.  Every thread of a workgroup
+// walks the same columns.
 __global__ __launch_bounds__(BX * BY) void kb_adv_v(const SimDesc* __restrict__ sims,
                                                     const StageTable* __restrict__ tab, int nsims)
 {
-    __shared__ dev::AdvWin<BX, kAdvTileY, kAdvTileZ, 3> win;
+    __shared__ dev::AdvRing<BX, BY, 4> ring;
     const int total = tab->start[nsims];
     const VbRange r = vb_range(total);
     for (int vb = r.begin; vb < r.end; vb += r.stride) {
@@ -117,10 +118,9 @@ __global__ __launch_bounds__(BX * BY) void kb_adv_v(const SimDesc* __restrict__ 
         const int gx = tab->gx[si], gy = tab->gy[si];
         const SimDesc& s = sims[si];
         const int n = s.now, p = 1 - s.now;
-        dev::advect_velocity_tile<BX, kAdvTileY, kAdvTileZ>(
+        dev::advect_velocity_march<BY, kAdvChunkZ>(
             s.g, s.u[n], s.v[n], s.w[n], s.u[p], s.v[p], s.w[p], s.flags, s.force.dt,
-            1 + BX * (lb % gx), 1 + kAdvTileY * ((lb / gx) % gy), 1 + kAdvTileZ * (lb / (gx * gy)),
-            win);
+            1 + BX * (lb % gx), 1 + BY * ((lb / gx) % gy), 1 + kAdvChunkZ * (lb / (gx * gy)), ring);
         __syncthreads();
     }
 }

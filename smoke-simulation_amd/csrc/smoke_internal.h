@@ -120,10 +120,9 @@ struct SimDesc {
 // divergence, gradient, density advection), single and batched.
 constexpr int kColZ = 4;
 
-// Cells per tile of the velocity advection (x spans one block row of
-// kBlockX cells): kAdvTileY rows x kAdvTileZ planes, staged in LDS with a
-// one-cell halo.
-constexpr int kAdvTileY = 4, kAdvTileZ = 2;
+// Planes per column of the velocity advection (a block of kBlockX x kBlockY
+// cells marches up z through kAdvChunkZ planes).
+constexpr int kAdvChunkZ = 16;
 
 // Work list of one batched stage: simulation s owns virtual blocks
 // [start[s], start[s+1]) arranged as a gx[s] x gy[s] x (rest) block grid.
@@ -161,8 +160,8 @@ hipError_t batch_jacobi(const SimDesc* sims, const StageTable* tab, int nsims, i
 hipError_t batch_flip(SimDesc* sims, int nsims, int pflip, hipStream_t st);
 // Resident blocks per CU of the batched kernels (for sizing persistent grids).
 int batch_cells_blocks_per_cu();
-// Velocity advection of every simulation in tiles (kBlockX x kAdvTileY x
-// kAdvTileZ cells over [1, n-1]^3, table built with those tiles).
+// Velocity advection of every simulation in columns (kBlockX x kBlockY x
+// kAdvChunkZ cells over [1, n-1]^3, table built with those columns).
 hipError_t batch_advect_velocity(const SimDesc* sims, const StageTable* tab, int nsims,
                                  int total_blocks, int grid, hipStream_t st);
 int batch_advect_velocity_blocks_per_cu();

@@ -522,27 +522,44 @@ __device__ __forceinline__ void gradient_at(const Grid& g, T* __restrict__ u, T*
 }
 
 // avgU/avgV/avgW (smokeSimulation.cu:409-447): 8-point means in the
-// reference's summation order.
+// reference's summation order.  The *_p forms take the index of the cell in
+// plane z (ic) and of the same cell in plane z-1 (im) separately, so that the
+// two planes can come from different slots of an LDS ring.
+template <typename T>
+__device__ __forceinline__ float avgU_p(const T* __restrict__ a, int im, int ic, int px)
+{
+    float t = ldf(a, im) + ldf(a, im + 1) + ldf(a, im - px) + ldf(a, im + 1 - px) + ldf(a, ic) +
+              ldf(a, ic + 1) + ldf(a, ic - px) + ldf(a, ic + 1 - px);
+    return t / 8.0f;
+}
+template <typename T>
+__device__ __forceinline__ float avgV_p(const T* __restrict__ a, int im, int ic, int px)
+{
+    float t = ldf(a, im) + ldf(a, im - 1) + ldf(a, im + px) + ldf(a, im - 1 + px) + ldf(a, ic) +
+              ldf(a, ic - 1) + ldf(a, ic + px) + ldf(a, ic - 1 + px);
+    return t / 8.0f;
+}
+template <typename T>
+__device__ __forceinline__ float avgW_p(const T* __restrict__ a, int im, int ic, int px)
+{
+    float t = ldf(a, ic) + ldf(a, ic - 1) + ldf(a, ic - px) + ldf(a, ic - 1 - px) + ldf(a, im) +
+              ldf(a, im - 1) + ldf(a, im - px) + ldf(a, im - 1 - px);
+    return t / 8.0f;
+}
 template <typename T>
 __device__ __forceinline__ float avgU(const T* __restrict__ a, int i, int px, int pl)
 {
-    float t = ldf(a, i - pl) + ldf(a, i + 1 - pl) + ldf(a, i - px - pl) + ldf(a, i + 1 - px - pl) +
-              ldf(a, i) + ldf(a, i + 1) + ldf(a, i - px) + ldf(a, i + 1 - px);
-    return t / 8.0f;
+    return avgU_p(a, i - pl, i, px);
 }
 template <typename T>
 __device__ __forceinline__ float avgV(const T* __restrict__ a, int i, int px, int pl)
 {
-    float t = ldf(a, i - pl) + ldf(a, i - 1 - pl) + ldf(a, i + px - pl) + ldf(a, i - 1 + px - pl) +
-              ldf(a, i) + ldf(a, i - 1) + ldf(a, i + px) + ldf(a, i - 1 + px);
-    return t / 8.0f;
+    return avgV_p(a, i - pl, i, px);
 }
 template <typename T>
 __device__ __forceinline__ float avgW(const T* __restrict__ a, int i, int px, int pl)
 {
-    float t = ldf(a, i) + ldf(a, i - 1) + ldf(a, i - px) + ldf(a, i - 1 - px) + ldf(a, i - pl) +
-              ldf(a, i - 1 - pl) + ldf(a, i - px - pl) + ldf(a, i - 1 - px - pl);
-    return t / 8.0f;
+    return avgW_p(a, i - pl, i, px);
 }
 
 // sampleSmoke (smokeSimulation.cu:451-484), split in two so that a kernel can
@@ -669,142 +686,164 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const T* __res
     if (doW) stf(w1, i, nw);
 }
 
-// LDS window of the tiled advection kernels: NF fields (as float; BF16
-// converts exactly) over the cells [x0-1, x0+AX] x [y0-1, y0+AY] x
-// [z0-1, z0+AZ] of a tile of AX x AY x AZ cells at (x0, y0, z0).
-template <int AX, int AY, int AZ, int NF>
-struct AdvWin {
-    static constexpr int WX = AX + 2, WY = AY + 2, WZ = AZ + 2, N = WX * WY * WZ;
-    float f[NF][N];
+// LDS ring of the z-marching velocity advection: NR planes of the window
+// [x0-1, x0+AX] x [y0-1, y0+AY] of u, v, w (as float; BF16 converts exactly),
+// window plane zw in slot zw % NR.
+template <int AX, int AY, int NR>
+struct AdvRing {
+    static constexpr int WX = AX + 2, WY = AY + 2, PN = WX * WY;
+    static constexpr int kPairs = PN / 2;  // WX is even: a pair never straddles a row
+    static constexpr int kPerThread = (kPairs + BX * BY - 1) / (BX * BY);
+    float f[3][NR * PN];
 };
 
-// Stages NF fields over the window in x-pairs (one 8-byte FP32 or 4-byte
-// BF16 load per pair; unrolled: every load of a thread is in flight at
-// once).  wx0 = x0 - 1 is even (tiles start at x0 = 1 + AX*m, AX even) and
-// WX is even, so a pair starts at an even x and never straddles a row; rows
-// hold px >= nx + 2 elements when nx is even (px is a multiple of 32), so the
-// pair of a window cell x <= nx lies inside its row.  Its second element may
-// be past nx: that cell is never read.  Pairs past (nx, ny, nz) are left at
-// zero; the rest lie inside the allocation.
-template <int AX, int AY, int AZ, int NF, typename T>
-__device__ __forceinline__ void stage_window(const Grid& g, const T* const (&src)[NF], int wx0,
-                                             int wy0, int wz0, AdvWin<AX, AY, AZ, NF>& win)
+// One window plane of u, v, w in registers, on its way to its ring slot: as
+// x-pairs (one 8-byte FP32 or 4-byte BF16 load each).  The window starts at
+// an even x (x0 - 1 = BX*m) and WX is even, so a pair starts at an even x and
+// never straddles a row; rows hold px >= nx + 2 elements when nx is even (px
+// is a multiple of 32), so the pair of a window cell x <= nx lies inside its
+// row.  Its second element may be past nx: that cell is never read.
+template <int AX, int AY, int NR>
+struct RingPlane {
+    float2 v[3][AdvRing<AX, AY, NR>::kPerThread];
+};
+
+template <int AX, int AY, int NR, typename T>
+__device__ __forceinline__ RingPlane<AX, AY, NR> ring_load(const Grid& g, const T* __restrict__ u,
+                                                           const T* __restrict__ v,
+                                                           const T* __restrict__ w, int wx0,
+                                                           int wy0, int zw)
 {
-    using W = AdvWin<AX, AY, AZ, NF>;
-    static_assert(AX % 2 == 0 && W::WX % 2 == 0, "x-pairs start at even x within a row");
-    constexpr int kPairs = W::N / 2;
+    using R = AdvRing<AX, AY, NR>;
     const int tid = (int)(threadIdx.x + BX * threadIdx.y);
-    constexpr int kLoads = (kPairs + BX * BY - 1) / (BX * BY);
+    RingPlane<AX, AY, NR> r;
 #pragma unroll
-    for (int it = 0; it < kLoads; ++it) {
-        const int k2 = tid + it * (BX * BY);
-        if (k2 >= kPairs) break;
-        const int k = 2 * k2;
-        const int lx = k % W::WX, r = k / W::WX, ly = r % W::WY, lz = r / W::WY;
-        const int xw = wx0 + lx, yw = wy0 + ly, zw = wz0 + lz;
-        float2 a[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) a[f] = make_float2(0.0f, 0.0f);
-        if (xw <= g.nx && yw <= g.ny && zw <= g.nz) {
+    for (int j = 0; j < R::kPerThread; ++j) {
+        const int k = 2 * (tid + j * (BX * BY));
+        const int lx = k % R::WX, ly = k / R::WX;
+        const int xw = wx0 + lx, yw = wy0 + ly;
+        r.v[0][j] = r.v[1][j] = r.v[2][j] = make_float2(0.0f, 0.0f);
+        if (k < R::PN && xw <= g.nx && yw <= g.ny) {
             const int i = g.idx(xw, yw, zw);
-#pragma unroll
-            for (int f = 0; f < NF; ++f) a[f] = ld2(src[f], i);
+            r.v[0][j] = ld2(u, i);
+            r.v[1][j] = ld2(v, i);
+            r.v[2][j] = ld2(w, i);
         }
+    }
+    return r;
+}
+
+template <int AX, int AY, int NR>
+__device__ __forceinline__ void ring_store(AdvRing<AX, AY, NR>& ring,
+                                           const RingPlane<AX, AY, NR>& r, int zw)
+{
+    using R = AdvRing<AX, AY, NR>;
+    const int tid = (int)(threadIdx.x + BX * threadIdx.y);
+    const int base = (zw % NR) * R::PN;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            win.f[f][k] = a[f].x;
-            win.f[f][k + 1] = a[f].y;
+    for (int j = 0; j < R::kPerThread; ++j) {
+        const int k = 2 * (tid + j * (BX * BY));
+        if (k < R::PN) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                ring.f[f][base + k] = r.v[f][j].x;
+                ring.f[f][base + k + 1] = r.v[f][j].y;
+            }
         }
     }
 }
 
-// One sample from the window when its four x-pairs lie inside it, else from
-// global memory: the same eight values either way.
-template <int WX, int WY, int WZ, typename T>
-__device__ __forceinline__ float tri_win(const T* __restrict__ a, const float* s, const Tri& t,
-                                         int wx0, int wy0, int wz0)
+// One sample from the ring when its corners lie in the window in x and y and
+// in the ring's planes [zlo, zlo + 2], else from global memory: the same
+// eight values either way.
+template <int WX, int WY, int PN, int NR, typename T>
+__device__ __forceinline__ float tri_ring(const T* __restrict__ a, const float* s, const Tri& t,
+                                          int wx0, int wy0, int zlo)
 {
-    const int lx = t.x0 - wx0, ly0 = t.y0 - wy0, ly1 = t.y1 - wy0, lz0 = t.z0 - wz0,
-              lz1 = t.z1 - wz0;
-    if (lx >= 0 && lx <= WX - 2 && ly0 >= 0 && ly1 <= WY - 1 && lz0 >= 0 && lz1 <= WZ - 1) {
-        const int l00 = lx + WX * (ly0 + WY * lz0), l10 = lx + WX * (ly1 + WY * lz0),
-                  l01 = lx + WX * (ly0 + WY * lz1), l11 = lx + WX * (ly1 + WY * lz1);
+    const int lx = t.x0 - wx0, ly0 = t.y0 - wy0, ly1 = t.y1 - wy0;
+    if (lx >= 0 && lx <= WX - 2 && ly0 >= 0 && ly1 <= WY - 1 && t.z0 >= zlo && t.z1 <= zlo + 2) {
+        const int p0 = (t.z0 % NR) * PN, p1 = (t.z1 % NR) * PN;
+        const int l00 = p0 + lx + WX * ly0, l10 = p0 + lx + WX * ly1, l01 = p1 + lx + WX * ly0,
+                  l11 = p1 + lx + WX * ly1;
         return tri_mix(t, make_float2(s[l00], s[l00 + 1]), make_float2(s[l10], s[l10 + 1]),
                        make_float2(s[l01], s[l01 + 1]), make_float2(s[l11], s[l11 + 1]));
     }
     return tri_eval(a, t);
 }
 
-// advect_velocity_at over one tile of AX x AY x AZ cells at (x0, y0, z0) with
-// a block of BX x BY threads (AX == BX): the block stages the window, then
-// each thread takes the cells (x0 + threadIdx.x, y0 + jy, z0 + jz) for
-// jy = threadIdx.y, threadIdx.y + BY, ... and reads the own components and
-// the averages (all inside the window: faces x+1, y+1, x-1, y-1, z-1) and
-// every sample whose corners fall inside it from LDS.  A back-trace longer
-// than one cell leaves the window and reads global memory.  Window cells past
-// (nx, ny, nz) are never read (averages reach x <= nx, y <= ny; clamped
-// corners stay below n) and are left at zero; every window cell is inside
-// the allocation otherwise.  Same values, same arithmetic as
-// advect_velocity_at: the results are identical.
-template <int AX, int AY, int AZ, typename T>
-__device__ __forceinline__ void advect_velocity_tile(const Grid& g, const T* __restrict__ u0,
-                                                     const T* __restrict__ v0,
-                                                     const T* __restrict__ w0,
-                                                     T* __restrict__ u1, T* __restrict__ v1,
-                                                     T* __restrict__ w1,
-                                                     const uint8_t* __restrict__ flags, float dt,
-                                                     int x0, int y0, int z0,
-                                                     AdvWin<AX, AY, AZ, 3>& win)
+// advect_velocity_at over a column of BX x AY cells at (x0, y0), z in
+// [z0, min(z0 + ZC, nz) - 1], with a block of BX x AY threads (AY == BY):
+// the block marches up z keeping window planes z-1, z, z+1 of u, v, w in a
+// 4-slot LDS ring; while plane z is computed, window plane z+2 is loaded into
+// registers and then stored into the slot of plane z-2 (last read in the
+// previous step, before the step's barrier).  Own components, averages and
+// in-window samples read LDS; a back-trace longer than one cell reads global
+// memory.  Each window plane is read from memory once per column, and a
+// thread has one cell per plane.  Window rows/columns past (ny, nx) are left
+// at zero and never read; window planes stay at or below nz.  Same values,
+// same arithmetic as advect_velocity_at: the results are identical.
+template <int AY, int ZC, typename T>
+__device__ __forceinline__ void advect_velocity_march(const Grid& g, const T* __restrict__ u0,
+                                                      const T* __restrict__ v0,
+                                                      const T* __restrict__ w0,
+                                                      T* __restrict__ u1, T* __restrict__ v1,
+                                                      T* __restrict__ w1,
+                                                      const uint8_t* __restrict__ flags, float dt,
+                                                      int x0, int y0, int z0,
+                                                      AdvRing<BX, AY, 4>& ring)
 {
-    static_assert(AX == BX && AY % BY == 0, "a row of the tile per wave, whole rows per thread");
-    constexpr int kRows = AY / BY;
-    static_assert(AZ * kRows <= 8, "the flags of a thread's cells fit one 64-bit word");
-    using W = AdvWin<AX, AY, AZ, 3>;
-    const int wx0 = x0 - 1, wy0 = y0 - 1, wz0 = z0 - 1;
-    const int x = x0 + (int)threadIdx.x;
-    // The flags of the thread's cells (cell c = jz * kRows + jr in byte c),
-    // loaded with the window; a cell outside [1, n-1] gets 0 (no work).
-    unsigned long long fls = 0;
-#pragma unroll
-    for (int jz = 0; jz < AZ; ++jz)
-#pragma unroll
-        for (int jr = 0; jr < kRows; ++jr) {
-            const int y = y0 + (int)threadIdx.y + jr * BY, z = z0 + jz;
-            if (x <= g.nx - 1 && y <= g.ny - 1 && z <= g.nz - 1)
-                fls |= (unsigned long long)flags[g.idx(x, y, z)] << (8 * (jz * kRows + jr));
-        }
-    const T* const src[3] = {u0, v0, w0};
-    stage_window(g, src, wx0, wy0, wz0, win);
+    static_assert(AY == BY, "one cell per thread and plane");
+    constexpr int NR = 4;
+    using R = AdvRing<BX, AY, NR>;
+    const int wx0 = x0 - 1, wy0 = y0 - 1;
+    const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
+    const int zlast = min(z0 + ZC, g.nz) - 1;  // the grid keeps z0 <= nz - 1
+    const bool col = x <= g.nx - 1 && y <= g.ny - 1;
+    {
+        const RingPlane<BX, AY, NR> a = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z0 - 1);
+        const RingPlane<BX, AY, NR> b = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z0);
+        const RingPlane<BX, AY, NR> c = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z0 + 1);
+        ring_store(ring, a, z0 - 1);
+        ring_store(ring, b, z0);
+        ring_store(ring, c, z0 + 1);
+    }
+    uint8_t fl_next = col ? flags[g.idx(x, y, z0)] : (uint8_t)0;
     __syncthreads();
-    if (fls == 0) return;
-    const float xf = (float)x;
-    for (int c = 0; c < AZ * kRows; ++c) {
-        const uint8_t fl = (uint8_t)(fls >> (8 * c));
-        const int y = y0 + (int)threadIdx.y + (c % kRows) * BY, z = z0 + c / kRows;
+    const float xf = (float)x, yf = (float)y;
+    const int l = (x - wx0) + R::WX * (y - wy0);
+    for (int z = z0; z <= zlast; ++z) {
+        const bool more = z < zlast;  // uniform
+        RingPlane<BX, AY, NR> nxt;
+        if (more) nxt = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z + 2);
+        const uint8_t fl = fl_next;
+        if (more && col) fl_next = flags[g.idx(x, y, z + 1)];
         const bool self = (fl & F_SELF) != 0;
         const bool doU = self && y < g.ny - 1 && z < g.nz - 1 && (fl & F_XM);
         const bool doV = self && x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM);
         const bool doW = self && x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM);
-        if (!(doU || doV || doW)) continue;
-        const int i = g.idx(x, y, z);
-        const int l = (x - wx0) + W::WX * ((y - wy0) + W::WY * (z - wz0));
-        const float uu = win.f[0][l], vv = win.f[1][l], ww = win.f[2][l];
-        const float aU = avgU(win.f[0], l, W::WX, W::WX * W::WY);
-        const float aV = avgV(win.f[1], l, W::WX, W::WX * W::WY);
-        const float aW = avgW(win.f[2], l, W::WX, W::WX * W::WY);
-        const float yf = (float)y, zf = (float)z;
-        const Tri tu = tri_setup(g, xf - dt * uu, (yf + 0.5f) - dt * aV,
-                                 (zf + 0.5f) - dt * aW, 0.0f, 0.5f, 0.5f);
-        const Tri tv = tri_setup(g, (xf + 0.5f) - dt * aU, yf - dt * vv,
-                                 (zf + 0.5f) - dt * aW, 0.5f, 0.0f, 0.5f);
-        const Tri tw = tri_setup(g, (xf + 0.5f) - dt * aU, (yf + 0.5f) - dt * aV,
-                                 zf - dt * ww, 0.5f, 0.5f, 0.0f);
-        const float nu = tri_win<W::WX, W::WY, W::WZ>(u0, win.f[0], tu, wx0, wy0, wz0);
-        const float nv = tri_win<W::WX, W::WY, W::WZ>(v0, win.f[1], tv, wx0, wy0, wz0);
-        const float nw = tri_win<W::WX, W::WY, W::WZ>(w0, win.f[2], tw, wx0, wy0, wz0);
-        if (doU) stf(u1, i, nu);
-        if (doV) stf(v1, i, nv);
-        if (doW) stf(w1, i, nw);
+        if (doU || doV || doW) {
+            const int pc = (z % NR) * R::PN + l, pm = ((z - 1) % NR) * R::PN + l;
+            const float uu = ring.f[0][pc], vv = ring.f[1][pc], ww = ring.f[2][pc];
+            const float aU = avgU_p(ring.f[0], pm, pc, R::WX);
+            const float aV = avgV_p(ring.f[1], pm, pc, R::WX);
+            const float aW = avgW_p(ring.f[2], pm, pc, R::WX);
+            const float zf = (float)z;
+            const Tri tu = tri_setup(g, xf - dt * uu, (yf + 0.5f) - dt * aV,
+                                     (zf + 0.5f) - dt * aW, 0.0f, 0.5f, 0.5f);
+            const Tri tv = tri_setup(g, (xf + 0.5f) - dt * aU, yf - dt * vv,
+                                     (zf + 0.5f) - dt * aW, 0.5f, 0.0f, 0.5f);
+            const Tri tw = tri_setup(g, (xf + 0.5f) - dt * aU, (yf + 0.5f) - dt * aV,
+                                     zf - dt * ww, 0.5f, 0.5f, 0.0f);
+            const float nu = tri_ring<R::WX, R::WY, R::PN, NR>(u0, ring.f[0], tu, wx0, wy0, z - 1);
+            const float nv = tri_ring<R::WX, R::WY, R::PN, NR>(v0, ring.f[1], tv, wx0, wy0, z - 1);
+            const float nw = tri_ring<R::WX, R::WY, R::PN, NR>(w0, ring.f[2], tw, wx0, wy0, z - 1);
+            const int i = g.idx(x, y, z);
+            if (doU) stf(u1, i, nu);
+            if (doV) stf(v1, i, nv);
+            if (doW) stf(w1, i, nw);
+        }
+        if (more) ring_store(ring, nxt, z + 2);
+        __syncthreads();
     }
 }
 

@@ -382,9 +382,9 @@ __global__ void k_gradient(Grid g, T* __restrict__ u, T* __restrict__ v, T* __re
                           blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
 }
 
-// Velocity advection tile: kAdvX x kAdvY x kAdvZ cells per block of BX x BY
-// threads (dev::advect_velocity_tile), in XCD tile order.
-constexpr int kAdvX = BX, kAdvY = kAdvTileY, kAdvZ = kAdvTileZ;
+// Velocity advection: a block of BX x BY threads marches a column of BX x BY
+// cells through kAdvZC planes (dev::advect_velocity_march), in XCD tile order.
+constexpr int kAdvZC = kAdvChunkZ;
 
 template <typename T>
 __global__ __launch_bounds__(BX * BY) void k_advect_velocity(
@@ -392,12 +392,12 @@ __global__ __launch_bounds__(BX * BY) void k_advect_velocity(
     T* __restrict__ u1, T* __restrict__ v1, T* __restrict__ w1, const uint8_t* __restrict__ flags,
     float dt)
 {
-    __shared__ dev::AdvWin<kAdvX, kAdvY, kAdvZ, 3> win;
+    __shared__ dev::AdvRing<BX, BY, 4> ring;
     const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
     const int L = xcd_tile_order();
-    dev::advect_velocity_tile<kAdvX, kAdvY, kAdvZ>(
-        g, u0, v0, w0, u1, v1, w1, flags, dt, 1 + kAdvX * (L % nbx), 1 + kAdvY * ((L / nbx) % nby),
-        1 + kAdvZ * (L / (nbx * nby)), win);
+    dev::advect_velocity_march<BY, kAdvZC>(g, u0, v0, w0, u1, v1, w1, flags, dt,
+                                           1 + BX * (L % nbx), 1 + BY * ((L / nbx) % nby),
+                                           1 + kAdvZC * (L / (nbx * nby)), ring);
 }
 
 template <typename T>
@@ -785,9 +785,8 @@ static hipError_t launch_advect_velocity(const Grid& g, const T* u0, const T* v0
                                          T* u1, T* v1, T* w1, const uint8_t* flags, float dt,
                                          hipStream_t st)
 {
-    const dim3 grid((unsigned)((g.nx - 1 + kAdvX - 1) / kAdvX),
-                    (unsigned)((g.ny - 1 + kAdvY - 1) / kAdvY),
-                    (unsigned)((g.nz - 1 + kAdvZ - 1) / kAdvZ));
+    const dim3 grid((unsigned)((g.nx - 1 + BX - 1) / BX), (unsigned)((g.ny - 1 + BY - 1) / BY),
+                    (unsigned)((g.nz - 1 + kAdvZC - 1) / kAdvZC));
     hipLaunchKernelGGL(k_advect_velocity<T>, grid, dim3(BX, BY), 0, st, g, u0, v0, w0, u1, v1, w1,
                        flags, dt);
     return hipGetLastError();
