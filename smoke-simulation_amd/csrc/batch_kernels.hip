@@ -109,7 +109,7 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
 __global__ __launch_bounds__(BX * BY) void kb_adv_v(const SimDesc* __restrict__ sims,
                                                     const StageTable* __restrict__ tab, int nsims)
 {
-    __shared__ dev::AdvRing<BX, BY, 4> ring;
+    __shared__ dev::AdvRing<BX, BY, 4, 3> ring;
     const int total = tab->start[nsims];
     const VbRange r = vb_range(total);
     for (int vb = r.begin; vb < r.end; vb += r.stride) {

@@ -686,58 +686,59 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const T* __res
     if (doW) stf(w1, i, nw);
 }
 
-// LDS ring of the z-marching velocity advection: NR planes of the window
-// [x0-1, x0+AX] x [y0-1, y0+AY] of u, v, w (as float; BF16 converts exactly),
-// window plane zw in slot zw % NR.
-template <int AX, int AY, int NR>
+// LDS ring of the z-marching advection kernels: NR planes of the window
+// [x0-XL, x0+AX+XL-1] x [y0-1, y0+AY] of NF fields (as float; BF16 converts
+// exactly), window plane zw in slot zw % NR.  x0 - XL is even (see
+// RingPlane).
+template <int AX, int AY, int NR, int NF, int XL = 1>
 struct AdvRing {
-    static constexpr int WX = AX + 2, WY = AY + 2, PN = WX * WY;
+    static constexpr int WX = AX + 2 * XL, WY = AY + 2, PN = WX * WY;
     static constexpr int kPairs = PN / 2;  // WX is even: a pair never straddles a row
     static constexpr int kPerThread = (kPairs + BX * BY - 1) / (BX * BY);
-    float f[3][NR * PN];
+    float f[NF][NR * PN];
 };
 
-// One window plane of u, v, w in registers, on its way to its ring slot: as
-// x-pairs (one 8-byte FP32 or 4-byte BF16 load each).  The window starts at
-// an even x (x0 - 1 = BX*m) and WX is even, so a pair starts at an even x and
-// never straddles a row; rows hold px >= nx + 2 elements when nx is even (px
-// is a multiple of 32), so the pair of a window cell x <= nx lies inside its
-// row.  Its second element may be past nx: that cell is never read.
-template <int AX, int AY, int NR>
+// One window plane in registers, on its way to its ring slot: as x-pairs
+// (one 8-byte FP32 or 4-byte BF16 load each).  The window starts at an even
+// x (x0 - XL = BX*m or BX*m - 2) and WX is even, so a pair starts at an even
+// x and never straddles a row; rows hold px >= nx + 2 elements when nx is
+// even (px is a multiple of 32), so the pair of a window cell x <= nx lies
+// inside its row.  Its second element may be past nx, and pairs left of
+// x = 0 are not loaded: such cells are never read.
+template <int AX, int AY, int NR, int NF, int XL>
 struct RingPlane {
-    float2 v[3][AdvRing<AX, AY, NR>::kPerThread];
+    float2 v[NF][AdvRing<AX, AY, NR, NF, XL>::kPerThread];
 };
 
-template <int AX, int AY, int NR, typename T>
-__device__ __forceinline__ RingPlane<AX, AY, NR> ring_load(const Grid& g, const T* __restrict__ u,
-                                                           const T* __restrict__ v,
-                                                           const T* __restrict__ w, int wx0,
-                                                           int wy0, int zw)
+template <int AX, int AY, int NR, int NF, int XL, typename T>
+__device__ __forceinline__ RingPlane<AX, AY, NR, NF, XL> ring_load(const Grid& g,
+                                                                   const T* const (&src)[NF],
+                                                                   int wx0, int wy0, int zw)
 {
-    using R = AdvRing<AX, AY, NR>;
+    using R = AdvRing<AX, AY, NR, NF, XL>;
     const int tid = (int)(threadIdx.x + BX * threadIdx.y);
-    RingPlane<AX, AY, NR> r;
+    RingPlane<AX, AY, NR, NF, XL> r;
 #pragma unroll
     for (int j = 0; j < R::kPerThread; ++j) {
         const int k = 2 * (tid + j * (BX * BY));
         const int lx = k % R::WX, ly = k / R::WX;
         const int xw = wx0 + lx, yw = wy0 + ly;
-        r.v[0][j] = r.v[1][j] = r.v[2][j] = make_float2(0.0f, 0.0f);
-        if (k < R::PN && xw <= g.nx && yw <= g.ny) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) r.v[f][j] = make_float2(0.0f, 0.0f);
+        if (k < R::PN && xw >= 0 && xw <= g.nx && yw <= g.ny) {
             const int i = g.idx(xw, yw, zw);
-            r.v[0][j] = ld2(u, i);
-            r.v[1][j] = ld2(v, i);
-            r.v[2][j] = ld2(w, i);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) r.v[f][j] = ld2(src[f], i);
         }
     }
     return r;
 }
 
-template <int AX, int AY, int NR>
-__device__ __forceinline__ void ring_store(AdvRing<AX, AY, NR>& ring,
-                                           const RingPlane<AX, AY, NR>& r, int zw)
+template <int AX, int AY, int NR, int NF, int XL>
+__device__ __forceinline__ void ring_store(AdvRing<AX, AY, NR, NF, XL>& ring,
+                                           const RingPlane<AX, AY, NR, NF, XL>& r, int zw)
 {
-    using R = AdvRing<AX, AY, NR>;
+    using R = AdvRing<AX, AY, NR, NF, XL>;
     const int tid = (int)(threadIdx.x + BX * threadIdx.y);
     const int base = (zw % NR) * R::PN;
 #pragma unroll
@@ -745,7 +746,7 @@ __device__ __forceinline__ void ring_store(AdvRing<AX, AY, NR>& ring,
         const int k = 2 * (tid + j * (BX * BY));
         if (k < R::PN) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
+            for (int f = 0; f < NF; ++f) {
                 ring.f[f][base + k] = r.v[f][j].x;
                 ring.f[f][base + k + 1] = r.v[f][j].y;
             }
@@ -790,19 +791,21 @@ __device__ __forceinline__ void advect_velocity_march(const Grid& g, const T* __
                                                       T* __restrict__ w1,
                                                       const uint8_t* __restrict__ flags, float dt,
                                                       int x0, int y0, int z0,
-                                                      AdvRing<BX, AY, 4>& ring)
+                                                      AdvRing<BX, AY, 4, 3>& ring)
 {
     static_assert(AY == BY, "one cell per thread and plane");
     constexpr int NR = 4;
-    using R = AdvRing<BX, AY, NR>;
+    using R = AdvRing<BX, AY, NR, 3>;
+    using P = RingPlane<BX, AY, NR, 3, 1>;
+    const T* const src[3] = {u0, v0, w0};
     const int wx0 = x0 - 1, wy0 = y0 - 1;
     const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
     const int zlast = min(z0 + ZC, g.nz) - 1;  // the grid keeps z0 <= nz - 1
     const bool col = x <= g.nx - 1 && y <= g.ny - 1;
     {
-        const RingPlane<BX, AY, NR> a = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z0 - 1);
-        const RingPlane<BX, AY, NR> b = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z0);
-        const RingPlane<BX, AY, NR> c = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z0 + 1);
+        const P a = ring_load<BX, AY, NR, 3, 1>(g, src, wx0, wy0, z0 - 1);
+        const P b = ring_load<BX, AY, NR, 3, 1>(g, src, wx0, wy0, z0);
+        const P c = ring_load<BX, AY, NR, 3, 1>(g, src, wx0, wy0, z0 + 1);
         ring_store(ring, a, z0 - 1);
         ring_store(ring, b, z0);
         ring_store(ring, c, z0 + 1);
@@ -813,8 +816,8 @@ __device__ __forceinline__ void advect_velocity_march(const Grid& g, const T* __
     const int l = (x - wx0) + R::WX * (y - wy0);
     for (int z = z0; z <= zlast; ++z) {
         const bool more = z < zlast;  // uniform
-        RingPlane<BX, AY, NR> nxt;
-        if (more) nxt = ring_load<BX, AY, NR>(g, u0, v0, w0, wx0, wy0, z + 2);
+        P nxt;
+        if (more) nxt = ring_load<BX, AY, NR, 3, 1>(g, src, wx0, wy0, z + 2);
         const uint8_t fl = fl_next;
         if (more && col) fl_next = flags[g.idx(x, y, z + 1)];
         const bool self = (fl & F_SELF) != 0;
@@ -895,6 +898,84 @@ __device__ __forceinline__ void advect_density_col(const Grid& g, const T* __res
     for (int j = 0; j < NZ; ++j)
         if (self[j]) stf(r1, g.idx(x, y, z0 + j), s[j]);
 }
+// advect_density_col over a column of BX x AY cells at (x0, y0), z in
+// [z0, min(z0 + ZC, nz - 1) - 1], with a block of BX x AY threads
+// (AY == BY; x0 a multiple of BX, lane x = cell x, lane 0 of the first
+// column idle), marching up z like advect_velocity_march: window planes z-1,
+// z, z+1 of the density (x in [x0-2, x0+BX+1], y in [y0-1, y0+AY]: the column
+// plus a one-cell halo, widened to an even start) in a 4-slot LDS ring, plane z+2 and the faces of the next cell loaded while
+// plane z is computed.  Samples whose corners lie in the ring read LDS, a
+// back-trace longer than one cell reads global memory.  Same values, same
+// arithmetic as advect_density_col: the results are identical.
+template <int AY, int ZC, typename T>
+__device__ __forceinline__ void advect_density_march(const Grid& g, const T* __restrict__ r0,
+                                                     T* __restrict__ r1, const T* __restrict__ u,
+                                                     const T* __restrict__ v,
+                                                     const T* __restrict__ w,
+                                                     const uint8_t* __restrict__ flags, float dt,
+                                                     int x0, int y0, int z0,
+                                                     AdvRing<BX, AY, 4, 1, 2>& ring)
+{
+    static_assert(AY == BY, "one cell per thread and plane");
+    constexpr int NR = 4;
+    using R = AdvRing<BX, AY, NR, 1, 2>;
+    using P = RingPlane<BX, AY, NR, 1, 2>;
+    const T* const src[1] = {r0};
+    const int wx0 = x0 - 2, wy0 = y0 - 1;
+    const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
+    const int zlast = min(z0 + ZC, g.nz - 1) - 1;  // the grid keeps z0 <= nz - 2
+    const bool col = x >= 1 && x <= g.nx - 2 && y <= g.ny - 2;
+    // Faces and flags of cell z (loaded one plane ahead).
+    struct Faces {
+        float u0, u1, v0, v1, w0, w1;
+        bool self;
+    };
+    auto faces = [&](int z) {
+        Faces f = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, false};
+        if (col) {
+            const int i = g.idx(x, y, z);
+            f.self = (flags[i] & F_SELF) != 0;
+            f.u0 = ldf(u, i);
+            f.u1 = ldf(u, i + 1);
+            f.v0 = ldf(v, i);
+            f.v1 = ldf(v, i + g.px);
+            f.w0 = ldf(w, i);
+            f.w1 = ldf(w, i + g.plane);
+        }
+        return f;
+    };
+    {
+        const P a = ring_load<BX, AY, NR, 1, 2>(g, src, wx0, wy0, z0 - 1);
+        const P b = ring_load<BX, AY, NR, 1, 2>(g, src, wx0, wy0, z0);
+        const P c = ring_load<BX, AY, NR, 1, 2>(g, src, wx0, wy0, z0 + 1);
+        ring_store(ring, a, z0 - 1);
+        ring_store(ring, b, z0);
+        ring_store(ring, c, z0 + 1);
+    }
+    Faces fn = faces(z0);
+    __syncthreads();
+    for (int z = z0; z <= zlast; ++z) {
+        const bool more = z < zlast;  // uniform
+        P nxt;
+        if (more) nxt = ring_load<BX, AY, NR, 1, 2>(g, src, wx0, wy0, z + 2);
+        const Faces fc = fn;
+        if (more) fn = faces(z + 1);
+        if (fc.self) {
+            float ut = (fc.u0 + fc.u1) / 2.0f;
+            float vt = (fc.v0 + fc.v1) / 2.0f;
+            float wt = (fc.w0 + fc.w1) / 2.0f;
+            float xt = (float)((double)x + 0.5 - (double)(ut * dt));
+            float yt = (float)((double)y + 0.5 - (double)(vt * dt));
+            float zt = (float)((double)z + 0.5 - (double)(wt * dt));
+            const Tri t = tri_setup(g, xt, yt, zt, 0.5f, 0.5f, 0.5f);
+            stf(r1, g.idx(x, y, z),
+                tri_ring<R::WX, R::WY, R::PN, NR>(r0, ring.f[0], t, wx0, wy0, z - 1));
+        }
+        if (more) ring_store(ring, nxt, z + 2);
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void advect_density_at(const Grid& g, const T* __restrict__ r0,
                                                   T* __restrict__ r1, const T* __restrict__ u,

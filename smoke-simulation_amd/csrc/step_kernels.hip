@@ -392,7 +392,7 @@ __global__ __launch_bounds__(BX * BY) void k_advect_velocity(
     T* __restrict__ u1, T* __restrict__ v1, T* __restrict__ w1, const uint8_t* __restrict__ flags,
     float dt)
 {
-    __shared__ dev::AdvRing<BX, BY, 4> ring;
+    __shared__ dev::AdvRing<BX, BY, 4, 3> ring;
     const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
     const int L = xcd_tile_order();
     dev::advect_velocity_march<BY, kAdvZC>(g, u0, v0, w0, u1, v1, w1, flags, dt,
@@ -400,13 +400,19 @@ __global__ __launch_bounds__(BX * BY) void k_advect_velocity(
                                            1 + kAdvZC * (L / (nbx * nby)), ring);
 }
 
+// Density advection: a block of BX x BY threads marches a column of BX x BY
+// cells through kAdvZC planes (dev::advect_density_march), in XCD tile order.
 template <typename T>
-__global__ void k_advect_density(Grid g, const T* __restrict__ r0, T* __restrict__ r1,
-                                 const T* __restrict__ u, const T* __restrict__ v,
-                                 const T* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
+__global__ __launch_bounds__(BX * BY) void k_advect_density(
+    Grid g, const T* __restrict__ r0, T* __restrict__ r1, const T* __restrict__ u,
+    const T* __restrict__ v, const T* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
 {
-    dev::advect_density_col<kZ>(g, r0, r1, u, v, w, flags, dt, blockIdx.x * BX + threadIdx.x,
-                                blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ + 1);
+    __shared__ dev::AdvRing<BX, BY, 4, 1, 2> ring;
+    const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+    const int L = xcd_tile_order();
+    dev::advect_density_march<BY, kAdvZC>(g, r0, r1, u, v, w, flags, dt, BX * (L % nbx),
+                                          1 + BY * ((L / nbx) % nby),
+                                          1 + kAdvZC * (L / (nbx * nby)), ring);
 }
 
 }  // namespace
@@ -808,8 +814,10 @@ template <typename T>
 static hipError_t launch_advect_density(const Grid& g, const T* r0, T* r1, const T* u, const T* v,
                                         const T* w, const uint8_t* flags, float dt, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_advect_density<T>, grid_col(g.nx - 1, g.ny - 2, g.nz - 2), dim3(BX, BY), 0,
-                       st, g, r0, r1, u, v, w, flags, dt);
+    const dim3 grid((unsigned)((g.nx - 1 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
+                    (unsigned)((g.nz - 2 + kAdvZC - 1) / kAdvZC));
+    hipLaunchKernelGGL(k_advect_density<T>, grid, dim3(BX, BY), 0, st, g, r0, r1, u, v, w, flags,
+                       dt);
     return hipGetLastError();
 }
 hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float* u,

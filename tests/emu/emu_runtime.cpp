@@ -132,7 +132,8 @@ void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 bloc
     // Kernels with barriers or lane exchanges run their threads concurrently.
     const bool threaded =
         strstr(name, "jacobi_fused") || strstr(name, "kb_jacobi<") || strstr(name, "k_ref_") ||
-        strstr(name, "k_proxy") || strstr(name, "k_advect_velocity") || strstr(name, "kb_adv_v");
+        strstr(name, "k_proxy") || strstr(name, "k_advect_velocity") || strstr(name, "kb_adv_v") ||
+        strstr(name, "k_advect_density");
     const unsigned nt = block.x * block.y * block.z;
     const EmuIdx gd{grid.x, grid.y, grid.z}, bd{block.x, block.y, block.z};
     auto set_thread = [&](unsigned t) {
