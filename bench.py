@@ -63,7 +63,7 @@ def describe(sc):
             f"{len(sc['sources'])} source(s), {proj} x{p['iterations']}, dt={sc['dt']}")
 
 
-BYTES_REF_SWEEP = 8            # FP16 sweep: e, qi, inv in; e out (2 B each)
+BYTES_REF_SWEEP = 7            # FP16 sweep: e, qi (2 B each), flags (1 B) in; e out (2 B)
 BYTES_REF_RESID = 19           # x, d (4), e (2), flags in; x, r out (4 each)
 BYTES_REF_NORM0 = 5            # d, flags in
 # BF16 fields: force 15, divergence 11, gradient 17, velocity advection 13,

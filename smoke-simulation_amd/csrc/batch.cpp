@@ -25,12 +25,13 @@ struct smoke_batch {
     hipStream_t stream = nullptr;
     char* dmem = nullptr;
     smk::SimDesc* d_sims = nullptr;
-    // [0] force, [1] faces 1..n-1, [2] interior, [3] jacobi, [4] velocity advection tiles
+    // [0] force, [1] faces 1..n-1, [2] interior, [3] jacobi, [4] velocity
+    // advection columns, [5] density advection columns
     smk::StageTable* d_tab = nullptr;
     smk::SourceTable* d_src = nullptr;
     std::vector<smk::SimDesc> h_sims;
-    smk::StageTable h_tab[5];
-    int totals[5] = {0, 0, 0, 0, 0};
+    smk::StageTable h_tab[6];
+    int totals[6] = {0, 0, 0, 0, 0, 0};
     int src_total = 0;
     int iterations = 0;
     float omega = 1.0f;
@@ -49,6 +50,7 @@ struct smoke_batch {
     // Resident workgroups per CU (queried outside graph capture).
     int occ_cells = 1;
     int occ_adv_v = 1;
+    int occ_adv_d = 1;
     int occ_jacobi[7] = {1, 1, 1, 1, 1, 1, 1};  // by sweeps per launch
 };
 
@@ -57,6 +59,7 @@ void refresh_occupancy(smoke_batch* b)
 {
     b->occ_cells = smk::batch_cells_blocks_per_cu();
     b->occ_adv_v = smk::batch_advect_velocity_blocks_per_cu();
+    b->occ_adv_d = smk::batch_advect_density_blocks_per_cu();
     for (int t = 1; t <= 6; ++t) b->occ_jacobi[t] = smk::batch_jacobi_blocks_per_cu(b->plan, t);
 }
 }  // namespace
@@ -73,7 +76,8 @@ void smk::batch_detach(smoke_batch* b, smoke_sim* s)
 
 namespace {
 
-constexpr int kTabForce = 0, kTabFaces = 1, kTabInterior = 2, kTabJacobi = 3, kTabAdvV = 4;
+constexpr int kTabForce = 0, kTabFaces = 1, kTabInterior = 2, kTabJacobi = 3, kTabAdvV = 4,
+              kTabAdvD = 5;
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
@@ -95,6 +99,10 @@ void fill_table(smk::StageTable& t, const std::vector<smoke_sim*>& sims, int kin
             gx = ceil_div(g.nx - 1, smk::kBlockX);
             gy = ceil_div(g.ny - 1, smk::kBlockY);
             gz = ceil_div(g.nz - 1, smk::kAdvChunkZ);
+        } else if (kind == kTabAdvD) {
+            gx = ceil_div(g.nx - 1, smk::kBlockX);  // x from 0
+            gy = ceil_div(g.ny - 2, smk::kBlockY);
+            gz = ceil_div(g.nz - 2, smk::kAdvChunkZ);
         } else {
             // Cells from x = 0 (the rows' line-aligned start; a stage starting
             // at x = 1 idles lane 0), y and z from 1 except for the force stage.
@@ -185,7 +193,10 @@ hipError_t enqueue_step(smoke_batch* b, int* sweep_launches)
         e = smk::batch_advect_velocity(
             b->d_sims, &b->d_tab[kTabAdvV], n, b->totals[kTabAdvV],
             persistent_grid(b->occ_adv_v, b->cus, b->totals[kTabAdvV]), b->stream);
-    if (e == hipSuccess) e = cells(smk::BS_ADV_D, kTabInterior, 0);
+    if (e == hipSuccess)
+        e = smk::batch_advect_density(
+            b->d_sims, &b->d_tab[kTabAdvD], n, b->totals[kTabAdvD],
+            persistent_grid(b->occ_adv_d, b->cus, b->totals[kTabAdvD]), b->stream);
     if (e == hipSuccess) e = smk::batch_flip(b->d_sims, n, L & 1, b->stream);
     *sweep_launches = L;
     return e;
@@ -323,6 +334,7 @@ smoke_status smoke_batch_create(smoke_sim* const* sims, int count, const float* 
     fill_table(b->h_tab[kTabInterior], b->sims, kTabInterior, b->plan, &b->totals[kTabInterior]);
     fill_table(b->h_tab[kTabJacobi], b->sims, kTabJacobi, b->plan, &b->totals[kTabJacobi]);
     fill_table(b->h_tab[kTabAdvV], b->sims, kTabAdvV, b->plan, &b->totals[kTabAdvV]);
+    fill_table(b->h_tab[kTabAdvD], b->sims, kTabAdvD, b->plan, &b->totals[kTabAdvD]);
     refresh_occupancy(b);
     // Source table: each source's bounding box, as the single-simulation
     // source fill computes it.

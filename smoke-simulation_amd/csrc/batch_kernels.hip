@@ -68,8 +68,9 @@ __global__ __launch_bounds__(BX * BY) void kb_sources(const SimDesc* __restrict_
     }
 }
 
-// Per-cell stages.  The coordinate offsets match the single-simulation
-// launches: force covers y from 1, the others start at 1 on every axis.
+// Per-cell stages (force, divergence, gradient), kColZ planes per thread.  The
+// coordinate offsets match the single-simulation launches: x from 0 (lane x
+// = cell x), y from 1, z from 0 for the force and from 1 for the others.
 template <int STAGE>
 __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ sims,
                                                     const StageTable* __restrict__ tab, int nsims,
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
         const int gx = tab->gx[si], gy = tab->gy[si];
         const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
         const SimDesc& s = sims[si];
-        const int n = s.now, p = 1 - s.now;
+        const int n = s.now;
         const int x = bx * BX + (int)threadIdx.x;
         const int y = by * BY + (int)threadIdx.y + 1;
         const int z0 = bz * kColZ + (STAGE == BS_FORCE ? 0 : 1);
@@ -92,12 +93,9 @@ __global__ __launch_bounds__(BX * BY) void kb_cells(const SimDesc* __restrict__ 
                                   z0);
         else if (STAGE == BS_DIV)
             dev::divergence_col<kColZ>(s.g, s.u[n], s.v[n], s.w[n], s.flags, s.d, x, y, z0);
-        else if (STAGE == BS_GRAD)
+        else
             dev::gradient_col<kColZ>(s.g, s.u[n], s.v[n], s.w[n], s.P[s.pcur ^ lp], s.flags, x, y,
                                      z0);
-        else
-            dev::advect_density_col<kColZ>(s.g, s.rho[n], s.rho[p], s.u[p], s.v[p], s.w[p],
-                                           s.flags, s.force.dt, x, y, z0);
     }
 }
 
@@ -121,6 +119,27 @@ __global__ __launch_bounds__(BX * BY) void kb_adv_v(const SimDesc* __restrict__ 
         dev::advect_velocity_march<BY, kAdvChunkZ>(
             s.g, s.u[n], s.v[n], s.w[n], s.u[p], s.v[p], s.w[p], s.flags, s.force.dt,
             1 + BX * (lb % gx), 1 + BY * ((lb / gx) % gy), 1 + kAdvChunkZ * (lb / (gx * gy)), ring);
+        __syncthreads();
+    }
+}
+
+// Density advection over every column of every simulation
+// (dev::advect_density_march), as kb_adv_v.
+__global__ __launch_bounds__(BX * BY) void kb_adv_d(const SimDesc* __restrict__ sims,
+                                                    const StageTable* __restrict__ tab, int nsims)
+{
+    __shared__ dev::AdvRing<BX, BY, 4, 1, 2> ring;
+    const int total = tab->start[nsims];
+    const VbRange r = vb_range(total);
+    for (int vb = r.begin; vb < r.end; vb += r.stride) {
+        const int si = find_owner(tab->start, nsims, vb);
+        const int lb = vb - tab->start[si];
+        const int gx = tab->gx[si], gy = tab->gy[si];
+        const SimDesc& s = sims[si];
+        const int n = s.now, p = 1 - s.now;
+        dev::advect_density_march<BY, kAdvChunkZ>(
+            s.g, s.rho[n], s.rho[p], s.u[p], s.v[p], s.w[p], s.flags, s.force.dt, BX * (lb % gx),
+            1 + BY * ((lb / gx) % gy), 1 + kAdvChunkZ * (lb / (gx * gy)), ring);
         __syncthreads();
     }
 }
@@ -233,7 +252,6 @@ hipError_t batch_cells(int stage, const SimDesc* sims, const StageTable* tab, in
     case BS_FORCE: hipLaunchKernelGGL(kb_cells<BS_FORCE>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     case BS_DIV: hipLaunchKernelGGL(kb_cells<BS_DIV>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     case BS_GRAD: hipLaunchKernelGGL(kb_cells<BS_GRAD>, gd, bd, 0, st, sims, tab, nsims, lp); break;
-    case BS_ADV_D: hipLaunchKernelGGL(kb_cells<BS_ADV_D>, gd, bd, 0, st, sims, tab, nsims, lp); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -268,7 +286,7 @@ hipError_t batch_flip(SimDesc* sims, int nsims, int pflip, hipStream_t st)
     return hipGetLastError();
 }
 
-int batch_cells_blocks_per_cu() { return occupancy(kb_cells<BS_ADV_D>, BX * BY); }
+int batch_cells_blocks_per_cu() { return occupancy(kb_cells<BS_GRAD>, BX * BY); }
 
 hipError_t batch_advect_velocity(const SimDesc* sims, const StageTable* tab, int nsims,
                                  int total_blocks, int grid, hipStream_t st)
@@ -279,6 +297,16 @@ hipError_t batch_advect_velocity(const SimDesc* sims, const StageTable* tab, int
 }
 
 int batch_advect_velocity_blocks_per_cu() { return occupancy(kb_adv_v, BX * BY); }
+
+hipError_t batch_advect_density(const SimDesc* sims, const StageTable* tab, int nsims,
+                                int total_blocks, int grid, hipStream_t st)
+{
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_adv_d, dim3((unsigned)grid), dim3(BX, BY), 0, st, sims, tab, nsims);
+    return hipGetLastError();
+}
+
+int batch_advect_density_blocks_per_cu() { return occupancy(kb_adv_d, BX * BY); }
 
 int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps)
 {

@@ -41,18 +41,6 @@ __device__ __forceinline__ void st_h2(uint32_t* __restrict__ p, int w, h2 v)
 }
 __device__ __forceinline__ float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 
-// Diagonal inverse per cell as FP16 (0 off the active set).
-__global__ void k_ref_inv(Grid g, const uint8_t* __restrict__ flags, uint16_t* __restrict__ inv)
-{
-    int x = blockIdx.x * BX + threadIdx.x;
-    int y = blockIdx.y * BY + threadIdx.y;
-    int z = blockIdx.z;
-    if (x >= g.px || y > g.ny || z > g.nz) return;
-    int i = g.idx(x, y, z);
-    const float v = (x < g.nx && y < g.ny && z < g.nz) ? dev::cell_inv(flags[i]) : 0.0f;
-    inv[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
-}
-
 // Block reduction of an FP64 sum and an FP32 max (blocks of BX*BY threads), in
 // a fixed order; thread 0 stores the block's pair.
 __device__ __forceinline__ void block_partials(double s2, float m, double* __restrict__ part,
@@ -302,6 +290,7 @@ __global__ __launch_bounds__(1024) void k_ref_finish(RefineCtl* __restrict__ ctl
 template <int TXP, int TY, int T>
 struct HalfLds {
     uint32_t v[T][2][TY + 2][TXP + 2];
+    uint16_t itab[256];  // FP16 diagonal inverse of each flags byte
 };
 
 // T fused FP16 Jacobi sweeps on the correction for one tile, temporal blocking
@@ -309,7 +298,12 @@ struct HalfLds {
 // owning a pair of adjacent columns (x0, x0+1), x0 even.  The grown tile is
 // 2*TXP cells wide; the inner 2*TXP - 2T columns and TY - 2T rows are written
 // (T even, so the inner region is whole pairs).  Pairs cover x in [0, px): the
-// cells beyond the interior have inv = 0 and r = 0, so they stay e = 0.  FIRST:
+// cells beyond the interior have inv = 0 and r = 0, so they stay e = 0.  The
+// diagonal inverse of a cell is the FP16 rounding of cell_inv(flags byte),
+// from a 256-entry LDS table: the lane loads its pair's two flags bytes (one
+// byte per cell instead of a two-byte FP16 stream).  Flags beyond the grid
+// (padding columns, row and plane) are 0 from creation on (only k_flags writes
+// flags, over the grid), so they give inv = 0 there.  FIRST:
 // e starts at 0 and the scaled right-hand side inv*r*s is formed from the FP32
 // r and written for the later launches of the refinement.
 template <int TXP, int TY, int T, bool FIRST>
@@ -317,7 +311,7 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
                                                          uint32_t* __restrict__ eout,
                                                          const float* __restrict__ q,
                                                          uint32_t* __restrict__ qi,
-                                                         const uint32_t* __restrict__ inv,
+                                                         const uint16_t* __restrict__ fl2,
                                                          const RefineCtl* __restrict__ ctl, int zc)
 {
     static_assert(T % 2 == 0, "pairs: the halo must be whole pairs");
@@ -327,6 +321,8 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
         uint32_t* flat = &sh.v[0][0][0][0];
         const int n = T * 2 * (TY + 2) * (TXP + 2);
         for (int k = (int)(threadIdx.x + TXP * threadIdx.y); k < n; k += TXP * TY) flat[k] = 0u;
+        for (int f = (int)(threadIdx.x + TXP * threadIdx.y); f < 256; f += TXP * TY)
+            sh.itab[f] = __builtin_bit_cast(uint16_t, (_Float16)dev::cell_inv((uint8_t)f));
     }
     __syncthreads();
     // XCD-contiguous tile order, as in the FP32 fused kernel.
@@ -363,14 +359,15 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
     // from them is computed when the plane is consumed, so no wait is forced
     // at the load.
     constexpr int D = dev::kPrefetch;
-    h2 ne[D], ni[D], nq[D];
+    h2 ne[D], nq[D];
+    uint16_t nfl[D];  // the pair's two flags bytes
     float2 nr[D];
     // Loads are unconditional (plane index clamped into the field, lanes
     // outside the grid read row 0); planes outside [0, nz] or past the chunk
     // read as 0 when consumed.
-    auto load = [&](int k, h2& e_, h2& i_, h2& q_, float2& r_) {
+    auto load = [&](int k, h2& e_, uint16_t& f_, h2& q_, float2& r_) {
         const int wi = wbase + min(max(k, 0), g.nz) * wplane;
-        i_ = ld_h2(inv, wi);
+        f_ = fl2[wi];
         if (FIRST) {
             e_ = zero2;
             q_ = zero2;
@@ -383,7 +380,7 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
     };
     auto valid = [&](int k) { return in_grid && k >= 0 && k <= g.nz && k < z1 + T; };
 #pragma unroll
-    for (int j = 0; j < D; ++j) load(z0 - T + j, ne[j], ni[j], nq[j], nr[j]);
+    for (int j = 0; j < D; ++j) load(z0 - T + j, ne[j], nfl[j], nq[j], nr[j]);
     // Unrolled by lcm(3, T+1) so the register windows rotate without copies;
     // the padded iterations load and write nothing.
     constexpr int U = dev::z_unroll(T);
@@ -393,7 +390,10 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
     for (int u = 0; u < U; ++u) {
         const int k = kb + u;
         const bool ok = valid(k);
-        const h2 p0 = ok ? ne[0] : zero2, ii = ok ? ni[0] : zero2;
+        const uint16_t fp = ok ? nfl[0] : (uint16_t)0;  // flags 0: inv = 0
+        const h2 ii = {__builtin_bit_cast(_Float16, sh.itab[fp & 0xff]),
+                       __builtin_bit_cast(_Float16, sh.itab[fp >> 8])};
+        const h2 p0 = ok ? ne[0] : zero2;
         h2 qv = ok ? nq[0] : zero2;
         if (FIRST) {
             const float2 rv = ok ? nr[0] : make_float2(0.0f, 0.0f);
@@ -403,11 +403,11 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
 #pragma unroll
         for (int j = 0; j + 1 < D; ++j) {
             ne[j] = ne[j + 1];
-            ni[j] = ni[j + 1];
+            nfl[j] = nfl[j + 1];
             nq[j] = nq[j + 1];
             nr[j] = nr[j + 1];
         }
-        load(k + D, ne[D - 1], ni[D - 1], nq[D - 1], nr[D - 1]);
+        load(k + D, ne[D - 1], nfl[D - 1], nq[D - 1], nr[D - 1]);
 #pragma unroll
         for (int t = T; t > 0; --t) {
             iq[t] = iq[t - 1];
@@ -482,30 +482,22 @@ hipError_t reduce_and_finish(const RefineBufs& b, int n, int stage, double tol, 
 }
 
 template <int T, bool FIRST>
-hipError_t launch_sweeps(const Grid& g, const RefineBufs& b, const float* q, int ein, int eout,
-                         int zc, hipStream_t st)
+hipError_t launch_sweeps(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
+                         int ein, int eout, int zc, hipStream_t st)
 {
+    // Two flags bytes per 16-bit word: the pair (x0, x0+1) of word x0/2.
+    const uint16_t* flags2 = reinterpret_cast<const uint16_t*>(flags);
     constexpr int TXP = 32, TY = 32;
     constexpr int OXC = 2 * TXP - 2 * T, OY = TY - 2 * T;
     dim3 grid((unsigned)((g.px + OXC - 1) / OXC), (unsigned)((g.ny - 2 + OY - 1) / OY),
               (unsigned)((g.nz - 2 + zc - 1) / zc));
     hipLaunchKernelGGL((k_ref_sweeps<TXP, TY, T, FIRST>), grid, dim3(TXP, TY), 0, st, g,
-                       (const uint32_t*)b.e[ein], b.e[eout], q, b.qi, (const uint32_t*)b.inv, b.ctl,
+                       (const uint32_t*)b.e[ein], b.e[eout], q, b.qi, flags2, b.ctl,
                        zc);
     return hipGetLastError();
 }
 
 }  // namespace
-
-hipError_t refine_build_inv(const Grid& g, const uint8_t* flags, const RefineBufs& b,
-                            hipStream_t st)
-{
-    dim3 grid((unsigned)((g.px + BX - 1) / BX), (unsigned)((g.ny + 1 + BY - 1) / BY),
-              (unsigned)(g.nz + 1));
-    hipLaunchKernelGGL(k_ref_inv, grid, dim3(BX, BY), 0, st, g, flags,
-                       reinterpret_cast<uint16_t*>(b.inv));
-    return hipGetLastError();
-}
 
 int refine_parts(const Grid& g)
 {
@@ -523,17 +515,17 @@ hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, con
     return reduce_and_finish(b, (int)(grid.x * grid.y * grid.z), 0, tol, st);
 }
 
-hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, int first, int ein,
-                         int eout, int sweeps, int zc, hipStream_t st)
+hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
+                         int first, int ein, int eout, int sweeps, int zc, hipStream_t st)
 {
     if (zc < 1) return hipErrorInvalidValue;
     switch (sweeps * 2 + (first ? 1 : 0)) {
-    case 4: return launch_sweeps<2, false>(g, b, q, ein, eout, zc, st);
-    case 5: return launch_sweeps<2, true>(g, b, q, ein, eout, zc, st);
-    case 8: return launch_sweeps<4, false>(g, b, q, ein, eout, zc, st);
-    case 9: return launch_sweeps<4, true>(g, b, q, ein, eout, zc, st);
-    case 12: return launch_sweeps<6, false>(g, b, q, ein, eout, zc, st);
-    case 13: return launch_sweeps<6, true>(g, b, q, ein, eout, zc, st);
+    case 4: return launch_sweeps<2, false>(g, b, q, flags, ein, eout, zc, st);
+    case 5: return launch_sweeps<2, true>(g, b, q, flags, ein, eout, zc, st);
+    case 8: return launch_sweeps<4, false>(g, b, q, flags, ein, eout, zc, st);
+    case 9: return launch_sweeps<4, true>(g, b, q, flags, ein, eout, zc, st);
+    case 12: return launch_sweeps<6, false>(g, b, q, flags, ein, eout, zc, st);
+    case 13: return launch_sweeps<6, true>(g, b, q, flags, ein, eout, zc, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -80,7 +80,7 @@ size_t refine_bytes(const Grid& g)
 {
     size_t e = (size_t)g.elems;
     size_t parts = (size_t)smk::refine_parts(g) + smk::kRefineStage2;
-    return align256(e * sizeof(float)) + 4 * align256(e * 2) + align256(sizeof(smk::RefineCtl)) +
+    return align256(e * sizeof(float)) + 3 * align256(e * 2) + align256(sizeof(smk::RefineCtl)) +
            align256(parts * sizeof(double)) + align256(parts * sizeof(unsigned));
 }
 
@@ -162,11 +162,6 @@ hipError_t smk::refresh_mask(smoke_sim* s, long long* launches)
     e = smk::build_flags(s->g, s->s, s->flags, s->stream);
     if (e != hipSuccess) return e;
     if (launches) *launches += 2;
-    if (s->refine_mem) {
-        e = smk::refine_build_inv(s->g, s->flags, s->rb, s->stream);
-        if (e != hipSuccess) return e;
-        if (launches) *launches += 1;
-    }
     if (s->owner) {
         e = smk::build_owner(s->g, s->owner, obst, s->prm.obstacle_mode, s->stream);
         if (e != hipSuccess) return e;
@@ -198,8 +193,6 @@ smoke_status ensure_refine(smoke_sim* s)
     p += align256(e * 2);
     b.qi = reinterpret_cast<uint32_t*>(p);
     p += align256(e * 2);
-    b.inv = reinterpret_cast<uint32_t*>(p);
-    p += align256(e * 2);
     b.ctl = reinterpret_cast<smk::RefineCtl*>(p);
     p += align256(sizeof(smk::RefineCtl));
     b.part = reinterpret_cast<double*>(p);
@@ -207,8 +200,6 @@ smoke_status ensure_refine(smoke_sim* s)
     b.pmax = reinterpret_cast<unsigned*>(p);
     s->refine_mem = m;
     hipError_t err = smk::fill_u8(reinterpret_cast<uint8_t*>(m), (long long)bytes, 0, s->stream);
-    s->mask_dirty = true;  // builds inv with the flags
-    if (err == hipSuccess) err = smk::refresh_mask(s, nullptr);
     if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
     if (err != hipSuccess) return fail(SMOKE_ERR_HIP, "refined solver: init failed: %s", hipGetErrorString(err));
     return SMOKE_OK;
@@ -318,7 +309,7 @@ hipError_t enqueue_refined(smoke_sim* s, long long* launches, long long* sweep_l
         for (int left = p.inner_sweeps; left > 0;) {
             const int t = std::min(left, T);
             const int eout = first ? 0 : 1 - ein;
-            e = smk::refine_sweeps(g, b, r == 0 ? s->d : b.q, first ? 1 : 0, ein, eout, t,
+            e = smk::refine_sweeps(g, b, r == 0 ? s->d : b.q, s->flags, first ? 1 : 0, ein, eout, t,
                                    s->plan.zc, s->stream);
             if (e != hipSuccess) return e;
             ein = eout;

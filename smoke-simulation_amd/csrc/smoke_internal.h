@@ -165,6 +165,11 @@ int batch_cells_blocks_per_cu();
 hipError_t batch_advect_velocity(const SimDesc* sims, const StageTable* tab, int nsims,
                                  int total_blocks, int grid, hipStream_t st);
 int batch_advect_velocity_blocks_per_cu();
+// Density advection of every simulation in columns (kBlockX x kBlockY x
+// kAdvChunkZ cells over [1, n-2]^3, x from 0; table built with those columns).
+hipError_t batch_advect_density(const SimDesc* sims, const StageTable* tab, int nsims,
+                                int total_blocks, int grid, hipStream_t st);
+int batch_advect_density_blocks_per_cu();
 int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps);
 
 // ------------------------------------------------------------ launchers
@@ -282,12 +287,12 @@ hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s
 
 // ------------------------------------------- mixed-precision pressure solve
 // Device buffers of the refinement loop (refine_kernels.hip), padded layout.
-// e, qi and inv hold FP16 values, two cells per 32-bit word.
+// e and qi hold FP16 values, two cells per 32-bit word.  The sweeps take the
+// diagonal inverse from the flags byte (a table in LDS).
 struct RefineBufs {
     float* q = nullptr;            // negated FP32 residual of the current refinement
     uint32_t* e[2] = {nullptr, nullptr};  // FP16 correction, ping-pong
     uint32_t* qi = nullptr;        // FP16 inv * r * s
-    uint32_t* inv = nullptr;       // FP16 diagonal inverse (0 off the active set)
     RefineCtl* ctl = nullptr;
     double* part = nullptr;        // per-block FP64 partial sums of r^2
     unsigned* pmax = nullptr;      // per-block max |r| (FP32 bits)
@@ -297,15 +302,13 @@ constexpr int kRefineStage2 = 1024;
 // Blocks (partials) of a residual pass over g; the partial buffers hold
 // refine_parts(g) + kRefineStage2 entries.
 int refine_parts(const Grid& g);
-hipError_t refine_build_inv(const Grid& g, const uint8_t* flags, const RefineBufs& b,
-                            hipStream_t st);
 // Norms of d (the residual of x = 0) and the step's control reset.
 hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, const RefineBufs& b,
                         double tol, hipStream_t st);
 // `sweeps` (2, 4 or 6) FP16 sweeps e[ein] -> e[eout]; `first`: e starts at 0
 // and the scaled right-hand side is formed from the FP32 q.
-hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, int first, int ein,
-                         int eout, int sweeps, int zc, hipStream_t st);
+hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
+                         int first, int ein, int eout, int sweeps, int zc, hipStream_t st);
 // x_new = x_old + e[ein]/s and the next residual (norms, control update for
 // refinement `stage`).
 hipError_t refine_resid(const Grid& g, const float* xo, float* xn, const float* d,
