@@ -147,3 +147,27 @@ def test_batch_64_workload_members_match_oracle():
         o.step(scs[k]["dt"], 2)
         assert np.array_equal(dens[k].view(np.uint32),
                               o.output_density().astype(np.float32).view(np.uint32)), scs[k]["name"]
+
+
+def test_batch_long_backtraces():
+    # The batched advection stages (velocity and density columns marched
+    # through an LDS ring) with back-traces longer than one cell, which read
+    # outside the ring: large velocity noise and dt, uneven sizes with partial
+    # columns on every axis.
+    scs = []
+    for k, dims in enumerate([(70, 13, 37), (33, 66, 20), (9, 8, 12)]):
+        sc = scenes.test_scene(dims, seed=60 + k, projection=smokesim.PROJ_JACOBI, iterations=5,
+                               noise=8.0)
+        sc["dt"] = 0.25 + 0.05 * k
+        scs.append(sc)
+    sims = [gpu_for(sc) for sc in scs]
+    with smokesim.Batch(sims, [sc["dt"] for sc in scs]) as b:
+        for _ in range(2):
+            b.step(use_graph=False)
+        b.synchronize()
+        states = [s.state() for s in sims]
+    for sc, st in zip(scs, states):
+        o = oracle_for(sc, 32)
+        o.step(sc["dt"], 2)
+        assert_bitwise_equal(st, o.state(), f"{sc['name']} long back-traces")
+
