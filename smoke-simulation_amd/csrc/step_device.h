@@ -688,10 +688,12 @@ __device__ __forceinline__ void advect_velocity_at(const Grid& g, const T* __res
 
 // LDS ring of the z-marching advection kernels: NR planes of the window
 // [x0-XL, x0+AX+XL-1] x [y0-1, y0+AY] of NF fields (as float; BF16 converts
-// exactly), window plane zw in slot zw % NR.  x0 - XL is even (see
+// exactly), window plane zw >= 0 in slot zw % NR (NR a power of two, taken
+// as zw & (NR - 1)).  x0 - XL is even (see
 // RingPlane).
 template <int AX, int AY, int NR, int NF, int XL = 1>
 struct AdvRing {
+    static_assert(NR > 0 && (NR & (NR - 1)) == 0, "ring slots: a power of two");
     static constexpr int WX = AX + 2 * XL, WY = AY + 2, PN = WX * WY;
     static constexpr int kPairs = PN / 2;  // WX is even: a pair never straddles a row
     static constexpr int kPerThread = (kPairs + BX * BY - 1) / (BX * BY);
@@ -740,7 +742,7 @@ __device__ __forceinline__ void ring_store(AdvRing<AX, AY, NR, NF, XL>& ring,
 {
     using R = AdvRing<AX, AY, NR, NF, XL>;
     const int tid = (int)(threadIdx.x + BX * threadIdx.y);
-    const int base = (zw % NR) * R::PN;
+    const int base = (zw & (NR - 1)) * R::PN;
 #pragma unroll
     for (int j = 0; j < R::kPerThread; ++j) {
         const int k = 2 * (tid + j * (BX * BY));
@@ -763,7 +765,7 @@ __device__ __forceinline__ float tri_ring(const T* __restrict__ a, const float* 
 {
     const int lx = t.x0 - wx0, ly0 = t.y0 - wy0, ly1 = t.y1 - wy0;
     if (lx >= 0 && lx <= WX - 2 && ly0 >= 0 && ly1 <= WY - 1 && t.z0 >= zlo && t.z1 <= zlo + 2) {
-        const int p0 = (t.z0 % NR) * PN, p1 = (t.z1 % NR) * PN;
+        const int p0 = (t.z0 & (NR - 1)) * PN, p1 = (t.z1 & (NR - 1)) * PN;
         const int l00 = p0 + lx + WX * ly0, l10 = p0 + lx + WX * ly1, l01 = p1 + lx + WX * ly0,
                   l11 = p1 + lx + WX * ly1;
         return tri_mix(t, make_float2(s[l00], s[l00 + 1]), make_float2(s[l10], s[l10 + 1]),
@@ -825,7 +827,7 @@ __device__ __forceinline__ void advect_velocity_march(const Grid& g, const T* __
         const bool doV = self && x < g.nx - 1 && z < g.nz - 1 && (fl & F_YM);
         const bool doW = self && x < g.nx - 1 && y < g.ny - 1 && (fl & F_ZM);
         if (doU || doV || doW) {
-            const int pc = (z % NR) * R::PN + l, pm = ((z - 1) % NR) * R::PN + l;
+            const int pc = (z & (NR - 1)) * R::PN + l, pm = ((z - 1) & (NR - 1)) * R::PN + l;
             const float uu = ring.f[0][pc], vv = ring.f[1][pc], ww = ring.f[2][pc];
             const float aU = avgU_p(ring.f[0], pm, pc, R::WX);
             const float aV = avgV_p(ring.f[1], pm, pc, R::WX);
