@@ -83,6 +83,48 @@ def test_bf16_store_is_round_to_nearest_even():
     g.close()
 
 
+def _rne_bf16_bits(a):
+    b = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16).astype(np.uint32)
+
+
+@pytest.mark.parametrize("dims,seed,noise,dt", [((70, 21, 19), 50, 0.5, 0.05),
+                                                 ((67, 18, 35), 51, 6.0, 0.3)])
+def test_bf16_advection_is_the_rounded_fp32_advection(dims, seed, noise, dt):
+    # BF16 fields widen to FP32 exactly and each output is rounded once, so
+    # the BF16 advection stages fed BF16-representable fields give, bit for
+    # bit, the BF16 rounding of the FP32 stages' outputs: the marched LDS ring,
+    # its x-pair loads and the global fallback of back-traces longer than one
+    # cell (the second case) included.
+    sc = scenes.test_scene(dims, seed=seed, projection=smokesim.PROJ_JACOBI, iterations=4)
+    sims = [gpu_for(sc), gpu_for(sc, storage=smokesim.STORAGE_BF16)]
+    for s in sims:
+        s.run_stage(smokesim.STAGE_SCENE, dt)
+    now = sims[0].index_now
+    assert sims[1].index_now == now
+    rng = np.random.default_rng(seed)
+    face, cell = sims[0].face_shape, sims[0].cell_shape
+    vel = [_rne_bf16_bits(rng.uniform(-noise, noise, face).astype(np.float32)).view(np.float32)
+           for _ in range(3)]
+    rho = _rne_bf16_bits(rng.uniform(0.0, 1.0, cell).astype(np.float32)).view(np.float32)
+    for s in sims:
+        for c in range(3):
+            s.set_velocity(c, now, vel[c])
+        s.set_density(now, rho)
+        s.run_stage(smokesim.STAGE_ADVECT_VEL, dt)
+    past = 1 - now
+    for c in range(3):
+        a32, a16 = sims[0].get_velocity(c, past), sims[1].get_velocity(c, past)
+        assert np.array_equal(a16.view(np.uint32), _rne_bf16_bits(a32)), c
+        sims[0].set_velocity(c, past, a16)  # the density stage reads the rounded faces
+    for s in sims:
+        s.run_stage(smokesim.STAGE_ADVECT_DENS, dt)
+    d32, d16 = sims[0].get_density(past), sims[1].get_density(past)
+    assert np.array_equal(d16.view(np.uint32), _rne_bf16_bits(d32))
+    for s in sims:
+        s.close()
+
+
 def test_refinement_reaches_the_tolerance():
     """At 32^3 the loop runs until ||r||/||div|| <= 1e-5 (about 2.3 n^2
     sweeps for Jacobi); the result matches FP64 Jacobi run for the same number
