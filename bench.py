@@ -207,7 +207,11 @@ def run_single(args, sc):
     smokesim.scenes.apply(sim, sc)
     dt = sc["dt"]
     tuned = None
-    if sc["params"]["projection"] == 2 and not args.no_autotune:
+    if args.plan:
+        t, tile, zc = (int(v) for v in args.plan.split(","))
+        sim.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
+        log(f"plan: {sim.get_jacobi_plan()}")
+    elif sc["params"]["projection"] == 2 and not args.no_autotune:
         plan, ms = sim.autotune()
         tuned = {"plan": plan, "ms_per_sweep": round(ms, 5)}
         log(f"autotune: {plan} at {ms * 1e3:.2f} us/sweep")
@@ -279,9 +283,12 @@ def run_single(args, sc):
             tr = json.load(f)
         key = f"{sc['name']}:{roofline['kernel']}"
         # Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-        # calibrated), valid only for the launch plan they were measured with.
-        if key in tr and tr[key].get("plan") == roofline["jacobi_plan"]:
-            roofline["traffic"] = tr[key]["bytes_per_launch"]
+        # calibrated), valid only for the launch plan they were measured with
+        # (an entry or a list of entries, one per measured plan).
+        entries = tr.get(key, [])
+        for e in entries if isinstance(entries, list) else [entries]:
+            if e.get("plan") == roofline["jacobi_plan"]:
+                roofline["traffic"] = e["bytes_per_launch"]
 
     step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
     out = {
@@ -559,6 +566,8 @@ def main():
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default Jacobi plan")
+    ap.add_argument("--plan", default=None,
+                    help="Jacobi plan T,tile,zc (e.g. 6,0,128) instead of the autotuned one")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32",
                     help="mixed: BF16 fields, REFINED projection (FP16 inner sweeps, 20 per "
