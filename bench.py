@@ -1,15 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark of the smoke time step on one MI355X.
 
-Default workload (grid_256): 256^3 cells, FP32, floor + seeded boxes covering
-5 % of cells, 4 seeded density sources, velocity noise +-0.01, net buoyancy
+Default workload (grid_512): 512^3 cells, FP32, floor + seeded boxes covering
+5 % of cells, 8 seeded density sources, velocity noise +-0.01, net buoyancy
 +9.8/density with a 0.001 density gate, clamp M = 4, dt = 0.05, and a
 pressure projection of 100 Jacobi sweeps.  Inputs are resident in HBM before
 the timed region; the timed region is K full steps (sources, mask, buoyancy,
 clamp, divergence, 100 sweeps, gradient, velocity and density advection)
-enqueued back to back and bracketed by stream synchronisations.
+enqueued back to back and bracketed by stream synchronisations.  The same
+inputs are then stepped in the mixed-precision form (reported beside the
+FP32 line, with its step-rate ratio).
 
-Other workloads: grid_512 (512^3, 8 sources), reference_80 (the reference's
+Other workloads: grid_256 (256^3, 4 sources), reference_80 (the reference's
 own 80^3 scene, red-black SOR on faces), batch_64 (64 simulations of uneven
 sizes in [32,160]^3, per-simulation dt, Jacobi x60, one launch per stage,
 replayed from a HIP graph), proxy_80 (the renderer's proxy-plane slicing
@@ -51,14 +53,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def describe(sc):
+def describe(sc, storage=0):
     p = sc["params"]
     nx, ny, nz = sc["dims"]
     proj = {0: "red-black SOR (reference, on faces)", 1: "red-black SOR (pressure)",
             2: "Jacobi", 3: f"refined (FP32 outer, FP16 inner x{p.get('inner_sweeps')} per "
                             "refinement, FP64 norms), total sweeps"}[p["projection"]]
     solid = "floor + seeded boxes 5%" if sc.get("solid") is not None else "floor"
-    fields = "BF16 fields, FP32 math" if p["projection"] == 3 else "FP32"
+    fields = {0: "FP32", 1: "BF16 fields, FP32 math", 2: "FP32 density, BF16 velocity, FP32 math"
+              }.get(storage, "FP32")
     return (f"{sc['name']}: {nx}x{ny}x{nz} {fields}, {solid}, {len(sc['obstacles'])} obstacle(s), "
             f"{len(sc['sources'])} source(s), {proj} x{p['iterations']}, dt={sc['dt']}")
 
@@ -90,30 +93,108 @@ def step_bytes_per_cell(params, storage=0):
     return BYTES_FIXED + BYTES_SWEEP * sweeps
 
 
-def cpu_baseline(sc, budget_s, max_steps):
-    """The oracle's FP32 step on this host's cores, on the same workload."""
+def host_info():
+    """CPU model and the cores this process may use (for the baseline)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cores": usable}
+
+
+def oracle_step_times(o, dt, threads, budget_s, max_steps, min_steps=1):
+    """Per-step wall times (s) of the oracle stepping `o` on `threads` threads
+    until `budget_s` or `max_steps` (at least `min_steps`)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    orc.set_threads(threads)
+    times, t0 = [], time.perf_counter()
+    while len(times) < max_steps:
+        t = time.perf_counter()
+        o.step(dt, 1)
+        times.append(time.perf_counter() - t)
+        if time.perf_counter() - t0 >= budget_s and len(times) >= min_steps:
+            break
+    return times
+
+
+def cpu_baseline(sc, budget_s, max_steps, single_budget_s):
+    """The oracle's FP32 step (the CPU restatement of the reference step) on
+    this host's cores, on the same workload: all usable cores (at most 16)
+    and one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     from smokesim import scenes
-    threads = min(16, os.cpu_count() or 1)
-    orc.set_threads(threads)
+    info = host_info()
+    threads = max(1, min(16, info["usable_cores"] or 1))
     nx, ny, nz = sc["dims"]
+    cells = nx * ny * nz
     o = orc.Oracle(nx, ny, nz, 32)
     scenes.apply(o, sc)
+    orc.set_threads(threads)
     o.step(sc["dt"], 1)  # warm-up
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        o.step(sc["dt"], 1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= max_steps:
-            break
-    cells = nx * ny * nz
-    return {
-        "value": cells * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-        "sample": f"{steps} full steps of the same {nx}x{ny}x{nz} workload after 1 warm-up step, "
-                  f"oracle FP32 restatement, OpenMP over z ({el:.1f} s)",
+    multi = oracle_step_times(o, sc["dt"], threads, budget_s, max_steps)
+    log(f"cpu baseline: {len(multi)} steps on {threads} threads")
+    single = oracle_step_times(o, sc["dt"], 1, single_budget_s, max_steps) if single_budget_s > 0 else []
+    orc.set_threads(threads)
+    out = {
+        "value": cells * len(multi) / sum(multi), "unit": "cell-updates/s", "cores": threads,
+        "kind": "port",
+        "sample": f"{len(multi)} full steps of the same {nx}x{ny}x{nz} workload after 1 warm-up "
+                  f"step, oracle FP32 restatement of the reference step, OpenMP over z on "
+                  f"{threads} threads ({sum(multi):.1f} s)",
+        "host": info,
     }
+    if single:
+        out["single_thread"] = {
+            "value": cells * len(single) / sum(single), "unit": "cell-updates/s", "cores": 1,
+            "sample": f"{len(single)} full step(s) of the same workload on 1 thread "
+                      f"({sum(single):.1f} s)"}
+    return out
+
+
+def reference_80_rates(args):
+    """Config c0, the reference's own 80^3 scene (red-black SOR on faces, 40
+    iterations, 100 steps): GPU step rate through the literal reference
+    projection, and the oracle's FP32 step on the host cores (all usable, and
+    one thread), each with p50/p99 step latency."""
+    import smokesim
+    from smokesim import scenes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    sc = scenes.reference_80()
+    nx, ny, nz = sc["dims"]
+    cells = nx * ny * nz
+    sim = smokesim.Simulation(nx, ny, nz)
+    scenes.apply(sim, sc)
+    sim.timing_enable(True)
+    for _ in range(sc["steps"]):
+        sim.step_async(sc["dt"])
+    sim.synchronize()
+    ms = sim.timing_step_ms()
+    sim.close()
+    out = {"workload": "reference_80: 80^3, floor + obstacle sphere r=10, source r=6 at "
+                       "(40,8,40), red-black SOR x40 on faces, dt=0.05, 100 steps",
+           "gpu": {"value": round(cells / (float(sum(ms)) / len(ms) * 1e-3), 1),
+                   "unit": "cell-updates/s", "latency_ms": latency(ms), "steps": len(ms)}}
+    if not args.no_cpu_baseline:
+        info = host_info()
+        threads = max(1, min(16, info["usable_cores"] or 1))
+        for name, th, budget in (("cpu", threads, 8.0), ("cpu_single_thread", 1, 8.0)):
+            o = orc.Oracle(nx, ny, nz, 32)
+            scenes.apply(o, sc)
+            t = oracle_step_times(o, sc["dt"], th, budget, sc["steps"])
+            out[name] = {"value": round(cells * len(t) / sum(t), 1), "unit": "cell-updates/s",
+                         "cores": th, "steps": len(t),
+                         "latency_ms": latency([1000.0 * x for x in t])}
+        orc.set_threads(threads)
+    return out
 
 
 def latency(ms):
@@ -122,7 +203,10 @@ def latency(ms):
             "p99": round(float(np.percentile(ms, 99)), 4)}
 
 
-MIXED = dict(projection=3, inner_sweeps=20)   # with BF16 fields
+# The mixed-precision form: field storage, projection and its label.
+MIXED = dict(projection=3, inner_sweeps=20)
+MIXED_STORAGE = 1
+MIXED_DTYPE = "bf16 fields, f16 inner sweeps, f32 math"
 
 
 def error_check(sc, steps, mixed_iterations):
@@ -187,39 +271,34 @@ def error_check(sc, steps, mixed_iterations):
     return out
 
 
-def run_single(args, sc):
+def time_form(args, sc, storage, extra=None, plan=None):
+    """Creates the simulation of `sc` with the given field storage (and extra
+    parameters), applies the Jacobi plan (`plan`, --plan, or the autotuner's
+    for the Jacobi projection), runs the warm-up steps, then times
+    args.steps full steps with HIP events per stage and around the sweep
+    loop.  Returns the measurements."""
     import numpy as np
     import smokesim
-    if args.projection:
-        sc["params"]["projection"] = {"reference": 0, "rbsor": 1, "jacobi": 2}[args.projection]
-    if args.iterations is not None:
-        sc["params"]["iterations"] = args.iterations
-    mixed = args.precision == "mixed"
-    storage = smokesim.STORAGE_BF16 if mixed else smokesim.STORAGE_FP32
     nx, ny, nz = sc["dims"]
-    cells = nx * ny * nz
-    fp32_iterations = sc["params"]["iterations"]
-    if mixed:
-        sc["params"].update(MIXED)
-    log("workload:", describe(sc))
-
     sim = smokesim.Simulation(nx, ny, nz, storage=storage)
     smokesim.scenes.apply(sim, sc)
+    if extra:
+        sim.set_params(**extra)
     dt = sc["dt"]
     tuned = None
-    if args.plan:
+    if plan is not None:
+        sim.set_jacobi_plan(**plan)
+    elif args.plan:
         t, tile, zc = (int(v) for v in args.plan.split(","))
         sim.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
         log(f"plan: {sim.get_jacobi_plan()}")
-    elif sc["params"]["projection"] == 2 and not args.no_autotune:
-        plan, ms = sim.autotune()
-        tuned = {"plan": plan, "ms_per_sweep": round(ms, 5)}
-        log(f"autotune: {plan} at {ms * 1e3:.2f} us/sweep")
+    elif sim.get_params()["projection"] == 2 and not args.no_autotune:
+        plan_, ms = sim.autotune()
+        tuned = {"plan": plan_, "ms_per_sweep": round(ms, 5)}
+        log(f"autotune: {plan_} at {ms * 1e3:.2f} us/sweep")
     for _ in range(args.warmup):
         sim.step_async(dt)
     sim.synchronize()
-    log(f"warm-up: {args.warmup} steps done")
-
     sim.timing_enable(True)
     sim.timing_reset()
     sim.synchronize()
@@ -235,65 +314,112 @@ def run_single(args, sc):
     dens = sim.read_density()
     if not np.isfinite(dens).all():
         raise SystemExit("non-finite density after the timed steps")
-    log(f"timed: {args.steps} steps in {wall:.3f} s; density sum {float(dens.sum()):.3f}")
+    norms = sim.step_norms()
+    p = sim.get_params()
+    res = {"wall": wall, "timing": timing, "step_ms": step_ms, "sweep_ms": sweep_ms,
+           "sweep_launches": sweep_launches, "params": p, "plan": sim.get_jacobi_plan(),
+           "autotune": tuned, "density_sum": float(dens.astype(np.float64).sum()),
+           "norms": {k: norms[k] for k in ("div_l2", "rhs_l2", "cfl", "nonfinite")},
+           "solve": sim.solve_info() if p["projection"] == 3 else None}
+    sim.close()
+    log(f"timed: {args.steps} steps in {wall:.3f} s; density sum {res['density_sum']:.3f}")
+    return res
 
-    ms_per_step = 1000.0 * wall / args.steps
-    p = sc["params"]
+
+def run_single(args, sc):
+    import smokesim
+    if args.projection:
+        sc["params"]["projection"] = {"reference": 0, "rbsor": 1, "jacobi": 2}[args.projection]
+    if args.iterations is not None:
+        sc["params"]["iterations"] = args.iterations
+    mixed = args.precision == "mixed"
+    nx, ny, nz = sc["dims"]
+    cells = nx * ny * nz
+    fp32_iterations = sc["params"]["iterations"]
+    log("workload:", describe(sc))
+    storage = MIXED_STORAGE if mixed else smokesim.STORAGE_FP32
+    r = time_form(args, sc, storage, dict(MIXED) if mixed else None)
+    ms_per_step = 1000.0 * r["wall"] / args.steps
+    p = r["params"]
+    timing = r["timing"]
+    steps_timed = max(timing["steps"], 1)
     bpc = step_bytes_per_cell(p, storage)
-    solve = sim.solve_info() if p["projection"] == 3 else None
 
-    # Dominant kernel: the pressure sweep.  Algorithmic bytes per launch =
-    # 13 B x interior cells (the sweep's domain) x sweeps per launch;
-    # duration from HIP events around the sweep loops on the simulation's
-    # stream.
+    # Dominant kernel: the pressure sweep launch.  Its compulsory bytes are
+    # what one launch must move however many sweeps it fuses: 13 B per
+    # interior cell (P in, d, flags, P out) for the fused Jacobi launch, 7 B
+    # (e in, scaled rhs, flags, e out) for the FP16 sweep launch.  Duration
+    # from HIP events around the sweep loops on the simulation's stream.
     interior = (nx - 2) * (ny - 2) * (nz - 2)
     roofline = None
-    if sweep_launches > 0:
-        avg_ms = sweep_ms / sweep_launches
+    launches_per_step = r["sweep_launches"] / steps_timed
+    if r["sweep_launches"] > 0:
+        avg_ms = r["sweep_ms"] / r["sweep_launches"]
+        sweeps_per_launch = None
         if p["projection"] == 3:
-            # The sweep loop also holds the residual passes; their share is
-            # counted with their own bytes (19 B/cell) so the figure stays the
-            # loop's algorithmic rate.
+            # The loop also holds the norm and residual passes, so the
+            # loop's compulsory bytes are averaged over its launches.
             refinements = -(-p["iterations"] // p["inner_sweeps"])
-            per_step = refinements * (p["inner_sweeps"] * BYTES_REF_SWEEP + BYTES_REF_RESID)
-            alg = per_step * interior * timing["steps"] / sweep_launches
+            t = r["plan"]["sweeps_per_launch"]
+            t = t if 2 <= t <= 6 and t % 2 == 0 else 4
+            per_ref = -(-p["inner_sweeps"] // t)
+            loop = BYTES_REF_NORM0 + refinements * (per_ref * BYTES_REF_SWEEP + BYTES_REF_RESID)
+            launches = 1 + refinements * (per_ref + 1)
+            comp = loop * interior / launches
+            unfused = (BYTES_REF_NORM0 + refinements * (p["inner_sweeps"] * BYTES_REF_SWEEP
+                                                        + BYTES_REF_RESID)) * interior / launches
             kernel = "k_ref_sweeps"
         elif p["projection"] == 2:
-            per_launch = p["iterations"] * timing["steps"] / sweep_launches
-            alg = BYTES_SWEEP * interior * per_launch
+            sweeps_per_launch = p["iterations"] / launches_per_step
+            comp = BYTES_SWEEP * interior
+            unfused = comp * sweeps_per_launch
             kernel = "k_jacobi_fused"
         elif p["projection"] == 1:
-            alg = BYTES_SWEEP * interior
+            comp = unfused = BYTES_SWEEP * interior
             kernel = "k_rbsor_half"
         else:
-            alg = BYTES_REF_HALF * interior
+            comp = unfused = BYTES_REF_HALF * interior
             kernel = "k_project_ref_half"
-        achieved = alg / (avg_ms * 1e-3) / 1e9
+        achieved = comp / (avg_ms * 1e-3) / 1e9
         roofline = {
             "bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2),
-            "alg_bytes_per_launch": int(alg), "launches_timed": sweep_launches,
-            "jacobi_plan": sim.get_jacobi_plan() if p["projection"] == 2 else None,
-            "autotune": tuned,
+            "alg_bytes_per_launch": int(comp),
+            "alg_bytes_model": "compulsory: 13 B x interior cells per fused launch (P in, d, "
+                               "flags, P out once, whatever the sweeps fused)"
+                               if kernel == "k_jacobi_fused" else "compulsory per launch",
+            "sweeps_per_launch": sweeps_per_launch,
+            "effective": {"bytes_per_launch": int(unfused),
+                          "frac": round(unfused / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "model": "unfused: 13 B x interior cells x sweeps per launch"},
+            "launches_timed": r["sweep_launches"],
+            "jacobi_plan": r["plan"] if p["projection"] == 2 else None,
+            "autotune": r["autotune"],
         }
-    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
-    if roofline and os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            tr = json.load(f)
-        key = f"{sc['name']}:{roofline['kernel']}"
-        # Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-        # calibrated), valid only for the launch plan they were measured with
-        # (an entry or a list of entries, one per measured plan).
-        entries = tr.get(key, [])
-        for e in entries if isinstance(entries, list) else [entries]:
-            if e.get("plan") == roofline["jacobi_plan"]:
-                roofline["traffic"] = e["bytes_per_launch"]
+        traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(traffic_file):
+            with open(traffic_file) as f:
+                tr = json.load(f)
+            # Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+            # calibrated) are a stored profile, valid only for the launch plan
+            # they were measured with.
+            entries = tr.get(f"{sc['name']}:{kernel}", [])
+            for e in entries if isinstance(entries, list) else [entries]:
+                if e.get("plan") == roofline["jacobi_plan"]:
+                    roofline["traffic"] = e["bytes_per_launch"]
+                    roofline["traffic_source"] = (f"stored rocprofv3 profile of this plan "
+                                                  f"(profiles/{e['source']}), not this run")
+                    roofline["traffic_over_compulsory"] = round(e["bytes_per_launch"] / comp, 3)
 
+    # Whole step: compulsory bytes with the launches the step makes (fixed
+    # stages once, one sweep's 13 B per fused launch) and the unfused model.
+    comp_bpc = (BYTES_FIXED + BYTES_SWEEP * launches_per_step) if p["projection"] == 2 else bpc
+    step_comp = cells * comp_bpc / (ms_per_step * 1e-3) / 1e9
     step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(cells * args.steps / wall, 1),
+        "value": round(cells * args.steps / r["wall"], 1),
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -302,31 +428,55 @@ def run_single(args, sc):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 fields, f16 inner sweeps, f32 math" if mixed else "f32",
+        "dtype": MIXED_DTYPE if mixed else "f32",
         "data": "synthetic",
         "config": {
-            "workload": describe(sc), "cells": cells, "projection": p["projection"],
+            "workload": describe(sc, storage), "cells": cells, "projection": p["projection"],
             "iterations": p["iterations"], "parallelism": "single-gpu",
             "precision": args.precision,
         },
-        "solve": ({"relative_residual": solve[0], "refinements": solve[1]} if solve else None),
-        "latency_ms": latency(step_ms),
-        "stage_ms_per_step": {k: round(v / max(timing["steps"], 1), 4)
-                              for k, v in timing["stage_ms"].items()},
-        "step_roofline": {"bytes_per_cell": bpc, "achieved": round(step_eff, 1),
+        "solve": ({"relative_residual": r["solve"][0], "refinements": r["solve"][1]}
+                  if r["solve"] else None),
+        "latency_ms": latency(r["step_ms"]),
+        "stage_ms_per_step": {k: round(v / steps_timed, 4) for k, v in timing["stage_ms"].items()},
+        "step_norms": r["norms"],
+        "step_roofline": {"bytes_per_cell": round(comp_bpc, 2), "achieved": round(step_comp, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(step_eff / HBM_PEAK_GBS, 4)},
+                          "frac": round(step_comp / HBM_PEAK_GBS, 4),
+                          "model": "compulsory bytes of the step's launches",
+                          "effective": {"bytes_per_cell": bpc, "achieved": round(step_eff, 1),
+                                        "frac": round(step_eff / HBM_PEAK_GBS, 4),
+                                        "model": "unfused streaming model (121 + 13 K B/cell)"}},
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    sim.close()
+    if not mixed and not args.no_mixed and p["projection"] == 2:
+        # The mixed-precision form on the same inputs, same sweep count.
+        log("mixed-precision form ...")
+        m = time_form(args, sc, MIXED_STORAGE, dict(MIXED, iterations=fp32_iterations),
+                      plan=dict(sweeps_per_launch=4, tile=0, zc=64))
+        mms = 1000.0 * m["wall"] / args.steps
+        out["mixed"] = {
+            "dtype": MIXED_DTYPE, "ms_per_step": round(mms, 4),
+            "value": round(cells * args.steps / m["wall"], 1),
+            "step_rate_vs_fp32": round(ms_per_step / mms, 4),
+            "latency_ms": latency(m["step_ms"]),
+            "stage_ms_per_step": {k: round(v / max(m["timing"]["steps"], 1), 4)
+                                  for k, v in m["timing"]["stage_ms"].items()},
+            "solve": ({"relative_residual": m["solve"][0], "refinements": m["solve"][1]}
+                      if m["solve"] else None),
+            "step_norms": m["norms"],
+        }
     if args.error_steps:
         esc = dict(sc, params=dict(sc["params"], projection=2, iterations=fp32_iterations))
-        out["accuracy"] = error_check(esc, args.error_steps, p["iterations"] if mixed else fp32_iterations)
+        out["accuracy"] = error_check(esc, args.error_steps, fp32_iterations)
+    if args.reference_80:
+        log("reference_80 (config c0) ...")
+        out["reference_80"] = reference_80_rates(args)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         bsc = dict(sc, params=dict(sc["params"], projection=2, iterations=fp32_iterations)) if mixed else sc
-        out["cpu_baseline"] = cpu_baseline(bsc, args.cpu_budget, 50)
+        out["cpu_baseline"] = cpu_baseline(bsc, args.cpu_budget, 50, args.cpu_single_budget)
     return out
 
 
@@ -560,7 +710,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="grid_256",
+    ap.add_argument("--workload", default="grid_512",
                     choices=["grid_256", "grid_512", "reference_80", "batch_64", "serve", "proxy_80"])
     ap.add_argument("--projection", choices=["jacobi", "rbsor", "reference"], default=None)
     ap.add_argument("--iterations", type=int, default=None)
@@ -569,6 +719,12 @@ def main():
     ap.add_argument("--plan", default=None,
                     help="Jacobi plan T,tile,zc (e.g. 6,0,128) instead of the autotuned one")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-single-budget", type=float, default=1.0,
+                    help="seconds of single-thread CPU baseline work (at least one step; 0: skip)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision form beside the FP32 line")
+    ap.add_argument("--reference-80", action="store_true",
+                    help="also time config c0 (reference_80) on the GPU and the host cores")
     ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32",
                     help="mixed: BF16 fields, REFINED projection (FP16 inner sweeps, 20 per "
                          "refinement, same total sweep count)")

@@ -36,7 +36,8 @@ typedef enum smoke_status {
     SMOKE_ERR_INVALID = 1, /* bad argument (dims, id, size, NULL) */
     SMOKE_ERR_HIP = 2,     /* HIP runtime error */
     SMOKE_ERR_NOMEM = 3,   /* device allocation failed */
-    SMOKE_ERR_STATE = 4    /* call not valid in the current state */
+    SMOKE_ERR_STATE = 4,   /* call not valid in the current state */
+    SMOKE_ERR_NONFINITE = 5 /* a field holds NaN or Inf (smoke_get_step_norms) */
 } smoke_status;
 
 /* Pressure projection.  REFERENCE is the reference's in-place red-black SOR on
@@ -302,6 +303,40 @@ smoke_status smoke_timing_sweeps(smoke_sim* sim, double* ms, long long* launches
 /* Step latencies (ms) of the last `cap` timed steps, oldest first; *count
  * receives how many were written. */
 smoke_status smoke_timing_step_ms(smoke_sim* sim, double* ms, int cap, int* count);
+
+/* ------------------------------------------------------------ step norms */
+/* Norms of the state a step leaves, reduced on the device in FP64 (sums) and
+ * FP32 (maxima) in a fixed order, so they depend only on the fields and the
+ * grid dims.  The reference computes none of them: it clamps face speeds
+ * without reporting them (smokeSimulation.cu:339-349) and checks only the
+ * final copy for errors (:815-818).  div_l2 is the velocity divergence over
+ * the cells the projection relaxes (flags bit 7) after the projection (half
+ * index_now, which advection reads and leaves alone), in the right-hand
+ * side's operand order (-u0 + u1 + -v0 + v1 + -w0 + w1, :379-381); rhs_l2 the
+ * same before it (the projection's right-hand side; REFERENCE computes it only
+ * while norms are enabled, else -1).  Speeds and the density are those of the
+ * step's output (half 1 - index_now); a component's faces on the grid walls
+ * of its own axis (u at x = 0 and x = nx, ...) are never written and are left
+ * out.  cfl = last dt x the largest speed: how many cells a face value's
+ * back-trace can span next step. */
+typedef struct smoke_step_norms {
+    double div_l2;         /* sqrt(sum div^2), after the projection        */
+    double rhs_l2;         /* sqrt(sum div^2), before it; -1 if unknown     */
+    double density_sum;    /* sum of the output density                     */
+    double cfl;            /* last dt * max(max_u, max_v, max_w)            */
+    float max_u, max_v, max_w; /* max |u|, |v|, |w| over the output faces   */
+    float max_density;     /* max |density|                                 */
+    long long nonfinite;   /* NaN or Inf values in output density, u, v, w  */
+    long long step;        /* steps enqueued when the norms were taken      */
+} smoke_step_norms;
+/* enable != 0: every following step ends with the reduction (two kernels on
+ * the simulation's stream, no synchronisation; counted in the density
+ * advection stage's timing), so norms cost no extra wait per step. */
+smoke_status smoke_norms_enable(smoke_sim* sim, int enable);
+/* Norms of the current state (the last step's, taken in-step when enabled,
+ * otherwise reduced now).  Waits for the simulation's work.  Returns
+ * SMOKE_ERR_NONFINITE, with *out filled, when nonfinite > 0. */
+smoke_status smoke_get_step_norms(smoke_sim* sim, smoke_step_norms* out);
 
 /* --------------------------------------------------------------- batches */
 /* Many independent simulations of different sizes advanced together: one

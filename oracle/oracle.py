@@ -234,6 +234,55 @@ class Oracle:
         return out
 
 
+# ------------------------------------------------------------------ step norms
+
+def divergence_f32(u, v, w):
+    """Per-cell divergence of face fields (nz+1, ny+1, nx+1) in FP32 with the
+    right-hand side's operand order, -u0 + u1 + -v0 + v1 + -w0 + w1, left to
+    right (project/smokeSimulation.cu:379-381); shape (nz, ny, nx)."""
+    f = np.float32
+    u = np.asarray(u, dtype=f)
+    v = np.asarray(v, dtype=f)
+    w = np.asarray(w, dtype=f)
+    u0, u1 = u[:-1, :-1, :-1], u[:-1, :-1, 1:]
+    v0, v1 = v[:-1, :-1, :-1], v[:-1, 1:, :-1]
+    w0, w1 = w[:-1, :-1, :-1], w[1:, :-1, :-1]
+    return ((((-u0 + u1) + -v0) + v1) + -w0) + w1
+
+
+def step_norms(sim, dt, rhs=None):
+    """Norms of a state (the reduction smoke_get_step_norms performs), from the
+    fields of `sim` (the oracle or a GPU simulation: get_density,
+    get_velocity, get_flags, index_now): divergence of half [now] and `rhs`
+    (the projection's right-hand side, or None) over relaxed cells, summed in
+    FP64; max |u|, |v|, |w| over the faces of the output half [1 - now] that
+    stages write (x >= 1 for u, y >= 1 for v, z >= 1 for w, within the cell
+    range); density total and max; non-finite count."""
+    now = sim.index_now
+    out_h = 1 - now
+    flags = sim.get_flags()
+    act = (flags & F_ACTIVE) != 0
+    vel_now = [sim.get_velocity(k, now) for k in range(3)]
+    div = divergence_f32(*vel_now).astype(np.float64)
+    rho = np.asarray(sim.get_density(out_h), dtype=np.float32)
+    uo, vo, wo = (np.asarray(sim.get_velocity(k, out_h), dtype=np.float32) for k in range(3))
+    faces = [uo[:-1, :-1, 1:-1], vo[:-1, 1:-1, :-1], wo[1:-1, :-1, :-1]]
+    res = {"div_l2": float(np.sqrt(np.sum(div[act] ** 2))),
+           "rhs_l2": -1.0 if rhs is None else float(np.sqrt(np.sum(
+               np.asarray(rhs, dtype=np.float64)[act] ** 2))),
+           "density_sum": float(np.sum(rho.astype(np.float64)))}
+    m = np.abs(rho[~np.isnan(rho)])
+    res["max_density"] = float(m.max()) if m.size else 0.0
+    bad = int(np.count_nonzero(~np.isfinite(rho)))
+    for name, a in zip(("max_u", "max_v", "max_w"), faces):
+        m = np.abs(a[~np.isnan(a)])
+        res[name] = float(m.max()) if m.size else 0.0
+        bad += int(np.count_nonzero(~np.isfinite(a)))
+    res["nonfinite"] = bad
+    res["cfl"] = float(np.float64(np.float32(dt)) * max(res["max_u"], res["max_v"], res["max_w"]))
+    return res
+
+
 # ------------------------------------------------ proxy-plane slicing geometry
 
 PROXY_MAX_VERTS_PER_PLANE = 13

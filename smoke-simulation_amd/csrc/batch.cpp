@@ -440,11 +440,14 @@ smoke_status smoke_batch_step(smoke_batch* b, int use_graph)
     b->pending.emplace_back(e0, e1);
     if (b->pending.size() > 256) HIP_TRY(collect(b));
     // Mirror the device-side parity flips.
-    for (smoke_sim* s : b->sims) {
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        smoke_sim* s = b->sims[i];
         std::lock_guard<std::mutex> lk(s->mu);
         s->now = 1 - s->now;
         s->pcur ^= (L & 1);
         s->steps_enqueued += 1;
+        s->rhs_valid = true;  // the batched divergence stage fills d
+        s->last_dt = b->dts[i];
     }
     return SMOKE_OK;
 }

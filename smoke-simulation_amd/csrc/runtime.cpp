@@ -234,6 +234,47 @@ smoke_status ensure_vort(smoke_sim* s)
     return SMOKE_OK;
 }
 
+// Allocates the step-norms records (once per simulation): the reduction's
+// scratch, then the public record it publishes.
+size_t norms_bytes() { return align256(smk::norms_scratch_bytes()) + sizeof(smoke_step_norms); }
+smoke_step_norms* norms_record(smoke_sim* s)
+{
+    return reinterpret_cast<smoke_step_norms*>(static_cast<char*>(s->norms_mem) +
+                                               align256(smk::norms_scratch_bytes()));
+}
+smoke_status ensure_norms(smoke_sim* s)
+{
+    if (s->norms_mem) return SMOKE_OK;
+    if (hipMalloc(&s->norms_mem, norms_bytes()) != hipSuccess) {
+        s->norms_mem = nullptr;
+        return fail(SMOKE_ERR_NOMEM, "norms: cannot allocate %zu bytes", norms_bytes());
+    }
+    return SMOKE_OK;
+}
+
+}  // namespace
+
+// Enqueues the norms reduction of the current state -- divergence of half
+// [now], output half [1 - now], right-hand side d if the last projection
+// left one -- with `scratch` for the partial records and the public record
+// written to `out` (device memory).
+hipError_t smk::enqueue_norms(smoke_sim* s, void* scratch, smoke_step_norms* out)
+{
+    const int n = s->now, p = 1 - s->now;
+    return smk::with_fields(s, [&](auto rho, auto u, auto v, auto w) {
+        return smk::step_norms(s->g, rho[p], u[p], v[p], w[p], u[n], v[n], w[n], s->d,
+                               s->rhs_valid ? 1 : 0, s->flags, scratch, out, s->last_dt,
+                               s->steps_enqueued, s->stream);
+    });
+}
+
+namespace {
+
+hipError_t enqueue_own_norms(smoke_sim* s)
+{
+    return smk::enqueue_norms(s, s->norms_mem, norms_record(s));
+}
+
 }  // namespace
 
 namespace {
@@ -367,6 +408,14 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
     const smoke_params& p = s->prm;
     hipError_t e = hipSuccess;
     if (p.projection == SMOKE_PROJ_REFERENCE) {
+        // The in-place form has no right-hand side of its own; with norms on,
+        // the divergence before the sweeps is kept for rhs_l2.
+        s->rhs_valid = s->norms_on;
+        if (s->norms_on) {
+            e = smk::divergence(g, s->u[n], s->v[n], s->w[n], s->flags, s->d, s->stream);
+            if (e != hipSuccess) return e;
+            *launches += 1;
+        }
         if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
         for (int it = 0; it < p.iterations; ++it)
             for (int o = 0; o < 2; ++o) {
@@ -382,6 +431,7 @@ hipError_t stage_project(smoke_sim* s, long long* launches)
     });
     if (e != hipSuccess) return e;
     *launches += 1;
+    s->rhs_valid = true;
     if (p.projection == SMOKE_PROJ_REFINED) {
         if ((e = sweep_mark(s, 0, 0)) != hipSuccess) return e;
         long long sweep_launches = 0;
@@ -439,9 +489,18 @@ hipError_t stage_advect_density(smoke_sim* s, float dt, long long* launches)
 {
     const int n = s->now, p = 1 - s->now;
     *launches += 1;
-    return smk::with_fields(s, [&](auto rho, auto u, auto v, auto w) {
+    hipError_t e = smk::with_fields(s, [&](auto rho, auto u, auto v, auto w) {
         return smk::advect_density(s->g, rho[n], rho[p], u[p], v[p], w[p], s->flags, dt, s->stream);
     });
+    if (e != hipSuccess || !s->norms_on) return e;
+    // The step's norms, reduced in-stream after its last stage (the record's
+    // step number is the one this step will have).
+    if (!s->norms_mem) return hipErrorNotInitialized;  // smoke_norms_enable allocates it
+    *launches += 3;
+    s->steps_enqueued += 1;
+    e = enqueue_own_norms(s);
+    s->steps_enqueued -= 1;
+    return e;
 }
 
 hipError_t run_stage(smoke_sim* s, int stage, float dt, long long* launches)
@@ -516,6 +575,7 @@ smoke_status step_locked(smoke_sim* s, float dt)
         HIP_TRY(hipEventRecord(se.ev[0], s->stream));
         s->cur = &se;
     }
+    s->last_dt = dt;
     for (int st = 0; st < SMOKE_STAGE_COUNT; ++st) {
         long long launches = 0;
         hipError_t e = run_stage(s, st, dt, &launches);
@@ -530,6 +590,7 @@ smoke_status step_locked(smoke_sim* s, float dt)
     }
     if (s->timing) s->pending.push_back(se);
     s->steps_enqueued += 1;
+    if (s->norms_on) s->norms_step = s->steps_enqueued;
     s->cur = nullptr;
     return SMOKE_OK;
 }
@@ -714,6 +775,7 @@ void smoke_destroy(smoke_sim* s)
     if (s->refine_mem) (void)hipFree(s->refine_mem);
     if (s->vort_mem) (void)hipFree(s->vort_mem);
     if (s->owner) (void)hipFree(s->owner);
+    if (s->norms_mem) (void)hipFree(s->norms_mem);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -892,6 +954,7 @@ smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
 {
     if (!s || !host) return fail(SMOKE_ERR_INVALID, "restore: NULL");
     std::lock_guard<std::mutex> lk(s->mu);
+    s->norms_step = -1;  // an in-step norms result no longer describes the state
     SMOKE_REFUSE_IN_BATCH(s, "restore");
     SnapshotHeader h;
     if (bytes < sizeof h) return fail(SMOKE_ERR_INVALID, "restore: blob too short");
@@ -911,6 +974,21 @@ smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
     if (h.prm.vorticity > 0.0f && (st = ensure_vort(s)) != SMOKE_OK) return st;
     if (h.prm.obstacle_velocity && (st = ensure_owner(s)) != SMOKE_OK) return st;
     const char* p = static_cast<const char*>(host) + sizeof h;
+    // The object list must satisfy what add_object enforces: type 0 or 1 and
+    // at most SMOKE_MAX_OBJECTS of each (the mask and boundary-condition
+    // sphere lists hold that many).
+    {
+        int per_type[2] = {0, 0};
+        for (int32_t k = 0; k < h.n_objects; ++k) {
+            Object o;
+            std::memcpy(&o, p + (size_t)k * sizeof(Object), sizeof o);
+            if (o.type != 0 && o.type != 1)
+                return fail(SMOKE_ERR_INVALID, "restore: object %d has unknown type %d", k, o.type);
+            if (++per_type[o.type] > SMOKE_MAX_OBJECTS)
+                return fail(SMOKE_ERR_INVALID, "restore: more than %d %s", SMOKE_MAX_OBJECTS,
+                            o.type == 0 ? "obstacles" : "sources");
+        }
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->objects.assign(reinterpret_cast<const Object*>(p), reinterpret_cast<const Object*>(p) + h.n_objects);
     p += (size_t)h.n_objects * sizeof(Object);
@@ -1095,6 +1173,7 @@ smoke_status smoke_set_solids(smoke_sim* s, const uint8_t* solid)
 {
     if (!s || !solid) return fail(SMOKE_ERR_INVALID, "set_solids: NULL");
     std::lock_guard<std::mutex> lk(s->mu);
+    s->norms_step = -1;  // an in-step norms result no longer describes the state
     SMOKE_REFUSE_IN_BATCH(s, "set_solids");
     size_t n = (size_t)s->g.nx * s->g.ny * s->g.nz;
     HIP_TRY(hipMemcpyAsync(s->staging, solid, n, hipMemcpyHostToDevice, s->stream));
@@ -1109,6 +1188,7 @@ smoke_status smoke_set_density(smoke_sim* s, int half, const float* src)
 {
     if (!s || !src || (half != 0 && half != 1)) return fail(SMOKE_ERR_INVALID, "set_density: bad argument");
     std::lock_guard<std::mutex> lk(s->mu);
+    s->norms_step = -1;  // an in-step norms result no longer describes the state
     SMOKE_REFUSE_IN_BATCH(s, "set_density");
     size_t bytes = (size_t)s->g.nx * s->g.ny * s->g.nz * sizeof(float);
     HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
@@ -1132,6 +1212,7 @@ smoke_status smoke_set_velocity(smoke_sim* s, int comp, int half, const float* s
     if ((half != 0 && half != 1) || comp < 0 || comp > 2)
         return fail(SMOKE_ERR_INVALID, "set_velocity: bad component %d or half %d", comp, half);
     std::lock_guard<std::mutex> lk(s->mu);
+    s->norms_step = -1;  // an in-step norms result no longer describes the state
     SMOKE_REFUSE_IN_BATCH(s, "set_velocity");
     size_t bytes = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1) * sizeof(float);
     HIP_TRY(hipMemcpyAsync(s->staging, src, bytes, hipMemcpyHostToDevice, s->stream));
@@ -1220,6 +1301,8 @@ smoke_status smoke_run_stage(smoke_sim* s, int stage, float dt)
     if (!std::isfinite(dt)) return fail(SMOKE_ERR_INVALID, "run_stage: dt must be finite");
     std::lock_guard<std::mutex> lk(s->mu);
     SMOKE_REFUSE_IN_BATCH(s, "run_stage");
+    s->norms_step = -1;
+    s->last_dt = dt;
     long long launches = 0;
     hipError_t e = run_stage(s, stage, dt, &launches);
     if (e != hipSuccess) return fail(SMOKE_ERR_HIP, "run_stage %d: %s", stage, hipGetErrorString(e));
@@ -1240,6 +1323,36 @@ smoke_status smoke_synchronize(smoke_sim* s)
     if (!s) return fail(SMOKE_ERR_INVALID, "synchronize: NULL sim");
     SMOKE_WAIT_BATCH(s);
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return SMOKE_OK;
+}
+
+smoke_status smoke_norms_enable(smoke_sim* s, int enable)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "norms_enable: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (enable) {
+        smoke_status st = ensure_norms(s);
+        if (st != SMOKE_OK) return st;
+    }
+    s->norms_on = enable != 0;
+    return SMOKE_OK;
+}
+
+smoke_status smoke_get_step_norms(smoke_sim* s, smoke_step_norms* out)
+{
+    if (!s || !out) return fail(SMOKE_ERR_INVALID, "get_step_norms: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    SMOKE_WAIT_BATCH(s);
+    smoke_status st = ensure_norms(s);
+    if (st != SMOKE_OK) return st;
+    // A batch step or a stage-level call leaves no in-step result for the
+    // current state: reduce now.
+    if (!(s->norms_on && s->norms_step == s->steps_enqueued)) HIP_TRY(enqueue_own_norms(s));
+    HIP_TRY(hipMemcpyAsync(out, norms_record(s), sizeof *out, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (out->nonfinite)
+        return fail(SMOKE_ERR_NONFINITE, "step %lld: %lld non-finite value(s) in density/velocity",
+                    out->step, out->nonfinite);
     return SMOKE_OK;
 }
 

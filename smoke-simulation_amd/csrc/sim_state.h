@@ -62,6 +62,15 @@ struct smoke_sim {
     // Moving-obstacle boundary condition: per-cell owner bytes (allocated
     // when obstacle_velocity is first set; built with the mask).
     uint8_t* owner = nullptr;
+    // Step norms: partial records and the result (allocated on first use).
+    // With norms_on every step ends with the reduction; norms_step is the
+    // value of steps_enqueued the stored result describes (-1: none).
+    // rhs_valid: d holds the right-hand side of the last projection stage.
+    void* norms_mem = nullptr;
+    bool norms_on = false;
+    long long norms_step = -1;
+    bool rhs_valid = false;
+    float last_dt = 0.0f;
     float* staging = nullptr;
     uint8_t* base = nullptr;
     uint8_t* s = nullptr;
@@ -120,6 +129,10 @@ auto with_fields(smoke_sim* s, Fn&& fn)
     if (s->storage == SMOKE_STORAGE_BF16) return fn(s->hrho, s->hu, s->hv, s->hw);
     return fn(s->rho, s->u, s->v, s->w);
 }
+// Enqueues the step-norms reduction of the simulation's current state on its
+// stream, with `scratch` (norms_scratch_bytes()) for the partial records and
+// the public record written to `out` (device memory).
+hipError_t enqueue_norms(smoke_sim* s, void* scratch, smoke_step_norms* out);
 // Rebuilds mask and flags if the base mask or an obstacle changed.
 hipError_t refresh_mask(smoke_sim* s, long long* launches);
 // Called by smoke_destroy for a simulation that is still in a batch: waits

@@ -285,6 +285,33 @@ hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s
                           const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
                           hipStream_t st);
 
+// ------------------------------------------------------------ step norms
+// Result record of the norms reduction (norms_kernels.hip), read back by the
+// host: FP64 sums of div(u[now])^2 and d^2 over relaxed cells and of the
+// output density; FP32 maxima of |u|, |v|, |w| over live faces of the output
+// half and of |density|; the count of non-finite output values.
+struct StepNorms {
+    double div2, rhs2, rho_sum;
+    float umax, vmax, wmax, rho_max;
+    unsigned long long nonfinite;
+};
+// Workgroups of the first reduction pass (at most): the scratch holds that
+// many partial records plus the reduced one (norms_scratch_bytes()).
+constexpr int kNormsBlocks = 1024;
+size_t norms_scratch_bytes();
+// Enqueues the reduction (partials, fixed-order total, then the public
+// record written to `out`, device memory).  rhs = 0: d is not read (no
+// right-hand side of this step is known; rhs_l2 = -1).  dt and step fill the
+// record's cfl and step.
+hipError_t step_norms(const Grid& g, const float* rho, const float* u, const float* v,
+                      const float* w, const float* un, const float* vn, const float* wn,
+                      const float* d, int rhs, const uint8_t* flags, void* scratch,
+                      smoke_step_norms* out, float dt, long long step, hipStream_t st);
+hipError_t step_norms(const Grid& g, const bf16s* rho, const bf16s* u, const bf16s* v,
+                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
+                      const float* d, int rhs, const uint8_t* flags, void* scratch,
+                      smoke_step_norms* out, float dt, long long step, hipStream_t st);
+
 // ------------------------------------------- mixed-precision pressure solve
 // Device buffers of the refinement loop (refine_kernels.hip), padded layout.
 // e and qi hold FP16 values, two cells per 32-bit word.  The sweeps take the

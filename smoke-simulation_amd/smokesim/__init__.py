@@ -20,6 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsmokesim.so")
 
 SMOKE_OK = 0
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_NONFINITE = 1, 2, 3, 4, 5
 PROJ_REFERENCE, PROJ_RBSOR, PROJ_JACOBI, PROJ_REFINED = 0, 1, 2, 3
 STORAGE_FP32, STORAGE_BF16 = 0, 1
 OBSTACLES_LAST_WINS, OBSTACLES_UNION = 0, 1
@@ -68,6 +69,19 @@ class JacobiPlan(ctypes.Structure):
 
 
 TILE_32x32, TILE_64x16 = 0, 1
+
+
+class StepNorms(ctypes.Structure):
+    _fields_ = [
+        ("div_l2", ctypes.c_double), ("rhs_l2", ctypes.c_double),
+        ("density_sum", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("max_u", ctypes.c_float), ("max_v", ctypes.c_float), ("max_w", ctypes.c_float),
+        ("max_density", ctypes.c_float),
+        ("nonfinite", ctypes.c_longlong), ("step", ctypes.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ProxyPlane(ctypes.Structure):
@@ -138,6 +152,8 @@ def _declare(lib):
         "smoke_timing_reset": (st, [vp]),
         "smoke_timing_step_ms": (st, [vp, dp, i, ip]),
         "smoke_timing_sweeps": (st, [vp, dp, llp]),
+        "smoke_norms_enable": (st, [vp, i]),
+        "smoke_get_step_norms": (st, [vp, ctypes.POINTER(StepNorms)]),
         "smoke_batch_create": (st, [ctypes.POINTER(vp), i, fp, ctypes.POINTER(vp)]),
         "smoke_batch_destroy": (None, [vp]),
         "smoke_batch_step": (st, [vp, i]),
@@ -449,6 +465,22 @@ class Simulation:
         n = ctypes.c_int()
         _check(self.lib.smoke_timing_step_ms(self.h, buf, cap, ctypes.byref(n)))
         return np.array(buf[: n.value], dtype=np.float64)
+
+    def norms_enable(self, on=True):
+        """Every following step ends with the norms reduction (in-stream)."""
+        _check(self.lib.smoke_norms_enable(self.h, 1 if on else 0))
+
+    def step_norms(self):
+        """Norms of the current state (smoke_get_step_norms) as a dict; a
+        non-finite field is reported in 'nonfinite' and 'status'
+        (ERR_NONFINITE), not raised."""
+        n = StepNorms()
+        st = self.lib.smoke_get_step_norms(self.h, ctypes.byref(n))
+        if st not in (SMOKE_OK, ERR_NONFINITE):
+            _check(st)
+        out = n.as_dict()
+        out["status"] = st
+        return out
 
     def state(self):
         """Every field of both halves, plus mask and flags (blocking)."""

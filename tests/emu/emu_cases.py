@@ -228,7 +228,26 @@ def case_advection():
         g.close()
 
 
-CASES = {"single": case_single, "plans": case_plans, "batch": case_batch,
+def case_norms():
+    # Step norms (two-pass fixed-order reduction with lane exchanges) against
+    # the oracle's restatement on the simulation's own fields.
+    import oracle as orc
+    sc = scenes.test_scene((70, 9, 11), seed=12, projection=2, iterations=5, noise=0.3)
+    g = gpu_for(sc)
+    g.norms_enable(True)
+    for _ in range(2):
+        g.step(sc["dt"], readback=False)
+    n = g.step_norms()
+    ref = orc.step_norms(g, sc["dt"])
+    for k in ("max_u", "max_v", "max_w", "max_density", "nonfinite"):
+        assert n[k] == ref[k], (k, n[k], ref[k])
+    for k in ("div_l2", "density_sum", "cfl"):
+        assert abs(n[k] - ref[k]) <= 1e-12 * abs(ref[k]), (k, n[k], ref[k])
+    assert n["rhs_l2"] > n["div_l2"]
+    g.close()
+
+
+CASES = {"single": case_single, "norms": case_norms, "plans": case_plans, "batch": case_batch,
          "server": case_server, "readback": case_readback, "precision": case_precision,
          "vorticity": case_vorticity, "advection": case_advection}
 

@@ -133,7 +133,7 @@ void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 bloc
     const bool threaded =
         strstr(name, "jacobi_fused") || strstr(name, "kb_jacobi<") || strstr(name, "k_ref_") ||
         strstr(name, "k_proxy") || strstr(name, "k_advect_velocity") || strstr(name, "kb_adv_v") ||
-        strstr(name, "k_advect_density") || strstr(name, "kb_adv_d");
+        strstr(name, "k_advect_density") || strstr(name, "kb_adv_d") || strstr(name, "k_norms");
     const unsigned nt = block.x * block.y * block.z;
     const EmuIdx gd{grid.x, grid.y, grid.z}, bd{block.x, block.y, block.z};
     auto set_thread = [&](unsigned t) {

@@ -90,3 +90,25 @@ int main() { return 0; }
     r = subprocess.run([gxx, "-std=c++11", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), path],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_compiled_caller_links_with_the_host_compiler_alone():
+    # The C++ caller of the nine functions (tests/drop_in/app_driver.cpp)
+    # builds with g++ and no HIP headers or flags, and resolves every symbol
+    # against libsmokesim.so (run on the GPU by test_gpu_drop_in.py).
+    gxx = shutil.which("g++")
+    lib = os.path.join(ROOT, "smoke-simulation_amd", "smokesim", "libsmokesim.so")
+    if gxx is None or not os.path.exists(lib):
+        pytest.skip("g++ or the built library not available")
+    out = os.path.join(tempfile.mkdtemp(prefix="smoke_dropin_"), "app_driver")
+    r = subprocess.run([gxx, "-std=c++11", "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "drop_in", "app_driver.cpp"),
+                        "-L", os.path.dirname(lib), "-lsmokesim", "-Wl,--no-undefined", "-o", out],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    nm = subprocess.run(["nm", "-u", "-C", out], capture_output=True, text=True).stdout
+    assert "hip" not in nm.lower()
+    for name in ("smoke_legacy_initialize", "smoke_legacy_simulate", "smoke_legacy_delete",
+                 "smoke_legacy_add_source", "smoke_legacy_add_obstacle"):
+        assert name in nm

@@ -36,7 +36,7 @@ def emu_lib(tmp_path_factory):
     return os.path.join(out, "libsmokesim_emu.so"), asan
 
 
-@pytest.mark.parametrize("case", ["single", "plans", "batch", "server", "readback", "vorticity",
+@pytest.mark.parametrize("case", ["single", "norms", "plans", "batch", "server", "readback", "vorticity",
                                   "advection"])
 def test_emulated_case_matches_oracle(emu_lib, case):
     lib, asan = emu_lib

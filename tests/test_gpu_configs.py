@@ -1,0 +1,116 @@
+"""The benchmark configurations end to end on the GPU.
+
+c0 (reference_80): the reference's own 80^3 scene -- floor, obstacle sphere
+r=10 at the centre, source r=6 at (40,8,40), velocity noise +-0.01, red-black
+SOR x40 on faces, dt 0.05 -- for its full 100 steps through the literal
+reference projection: the output density equals the FP32 oracle's bit for bit
+after every step, every field of both halves every 10 steps, and after 100
+steps the FP32 result is within the stated tolerance of the FP64 oracle (mask
+and flags bit-exact).
+
+c2 (grid_512): 512^3, 5 % boxes, 8 sources, Jacobi x100, with the plan the
+autotuner picks at that size: two FP32 steps equal the FP32 oracle bit for
+bit, field by field; then the mixed-precision form from the same inputs
+stays within the stated tolerance of the FP32 form, with mask and flags
+bit-exact."""
+import numpy as np
+import pytest
+
+import smokesim
+from helpers import assert_bitwise_equal, bit_mismatch, gpu_for, oracle_for, rel_l2
+from smokesim import scenes
+
+pytestmark = pytest.mark.gpu
+
+# FP32 (GPU = FP32 oracle, bit for bit) against the FP64 oracle after the 100
+# steps of c0: measured 8.6e-7 (density) and <= 8.4e-7 (velocity) relative
+# L2, 4.8e-6 max-abs density.
+C0_REL_L2 = 1e-5
+C0_MAX_ABS = 1e-4
+
+
+def test_c0_reference_80_hundred_steps():
+    sc = scenes.reference_80()
+    assert sc["params"]["projection"] == smokesim.PROJ_REFERENCE
+    assert sc["params"]["iterations"] == 40 and sc["steps"] == 100
+    g = gpu_for(sc)
+    o32 = oracle_for(sc, 32)
+    o64 = oracle_for(sc, 64)
+    assert np.array_equal(g.get_mask(), o32.get_mask())
+    for k in range(sc["steps"]):
+        out = g.step(sc["dt"])
+        o32.step(sc["dt"])
+        o64.step(sc["dt"])
+        n = bit_mismatch(out, o32.output_density())
+        assert n == 0, f"step {k + 1}: {n} density values differ from the FP32 oracle"
+        if (k + 1) % 10 == 0:
+            assert_bitwise_equal(g.state(), o32.state(), f"c0 step {k + 1}")
+    h = 1 - g.index_now
+    assert np.array_equal(g.get_mask(), o64.get_mask())
+    assert np.array_equal(g.get_flags(), o64.get_flags())
+    assert rel_l2(g.get_density(h), o64.get_density(h)) <= C0_REL_L2
+    assert np.abs(g.get_density(h) - o64.get_density(h)).max() <= C0_MAX_ABS
+    for c in range(3):
+        assert rel_l2(g.get_velocity(c, h), o64.get_velocity(c, h)) <= C0_REL_L2
+    d = g.get_density(h)
+    assert d.min() >= 0.0 and d.max() <= 1.0 + 1e-6 and d.sum() > 100.0   # weights round to 1 + ulp
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def scene512():
+    return scenes.grid_512()
+
+
+def test_c2_grid_512_two_fp32_steps_bitwise(scene512):
+    sc = scene512
+    g = gpu_for(sc)
+    plan, _ = g.autotune()
+    o = oracle_for(sc, 32)
+    assert np.array_equal(g.get_mask(), o.get_mask())
+    assert np.array_equal(g.get_flags(), o.get_flags())
+    for _ in range(2):
+        g.step_async(sc["dt"])
+    g.synchronize()
+    o.step(sc["dt"], 2)
+    assert g.index_now == o.index_now
+    # field by field (each 540 MB): the full state at once would hold ~20 GB
+    for h in (0, 1):
+        n = bit_mismatch(g.get_density(h), o.get_density(h))
+        assert n == 0, f"density half {h}: {n} values differ (plan {plan})"
+        for c in range(3):
+            n = bit_mismatch(g.get_velocity(c, h), o.get_velocity(c, h))
+            assert n == 0, f"velocity {c} half {h}: {n} values differ (plan {plan})"
+    g.close()
+
+
+# Mixed-precision form against the FP32 form after 2 steps at 512^3 (same
+# inputs, same sweep count).
+C2_MIXED_DENSITY_REL_L2 = 1e-2
+C2_MIXED_VELOCITY_REL_L2 = 5e-2
+
+
+def test_c2_grid_512_mixed_form_within_tolerance(scene512):
+    sc = scene512
+    ref = gpu_for(sc)
+    for _ in range(2):
+        ref.step_async(sc["dt"])
+    ref.synchronize()
+    m = gpu_for(sc, storage=smokesim.STORAGE_BF16)
+    m.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=20, iterations=100)
+    for _ in range(2):
+        m.step_async(sc["dt"])
+    m.synchronize()
+    assert np.array_equal(m.get_mask(), ref.get_mask())
+    assert np.array_equal(m.get_flags(), ref.get_flags())
+    h = 1 - ref.index_now
+    assert m.index_now == ref.index_now
+    e = rel_l2(m.get_density(h), ref.get_density(h))
+    assert e <= C2_MIXED_DENSITY_REL_L2, e
+    for c in range(3):
+        e = rel_l2(m.get_velocity(c, h), ref.get_velocity(c, h))
+        assert e <= C2_MIXED_VELOCITY_REL_L2, (c, e)
+    n = m.step_norms()
+    assert n["status"] == smokesim.SMOKE_OK and n["nonfinite"] == 0
+    m.close()
+    ref.close()

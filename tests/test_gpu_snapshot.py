@@ -57,3 +57,31 @@ def test_restore_refuses_mismatch_and_corruption():
         a.restore(bytes(bad))
     a.restore(blob)  # the intact blob still loads
     a.close()
+
+
+def test_restore_refuses_corrupt_object_lists():
+    # A blob whose object list breaks what add_object enforces (more than
+    # SMOKE_MAX_OBJECTS of a type, or an unknown type) is refused before any
+    # state changes; the intact blob still loads.
+    dims = (16, 12, 10)
+    a = smokesim.Simulation(*dims)
+    for k in range(smokesim.MAX_OBJECTS):
+        a.add_obstacle(1.0 + k % 8, 2.0 + k // 8, 3.0, 1.0)
+    a.add_source(8.0, 3.0, 5.0, 2.0)
+    blob = a.snapshot()
+    n = smokesim.MAX_OBJECTS + 1
+    head = len(blob) - smokesim.footprint_bytes(*dims) - 32 * n   # objects are 32 bytes
+    src_type = head + 32 * (n - 1) + 28
+    assert blob[src_type:src_type + 4] == (1).to_bytes(4, "little")
+    bad = bytearray(blob)
+    bad[src_type:src_type + 4] = (0).to_bytes(4, "little")       # 65 obstacles
+    with pytest.raises(smokesim.SmokeError):
+        a.restore(bytes(bad))
+    bad[src_type:src_type + 4] = (7).to_bytes(4, "little")       # unknown type
+    with pytest.raises(smokesim.SmokeError):
+        a.restore(bytes(bad))
+    a.restore(blob)
+    again = a.snapshot()
+    assert again[head:head + 32 * n] == blob[head:head + 32 * n]   # the object list as saved
+    assert a.get_mask().sum() < 16 * 11 * 10                       # the obstacles are in the mask
+    a.close()
