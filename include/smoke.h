@@ -337,6 +337,20 @@ smoke_status smoke_norms_enable(smoke_sim* sim, int enable);
  * otherwise reduced now).  Waits for the simulation's work.  Returns
  * SMOKE_ERR_NONFINITE, with *out filled, when nonfinite > 0. */
 smoke_status smoke_get_step_norms(smoke_sim* sim, smoke_step_norms* out);
+/* smoke_get_step_norms for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * reference counterpart.  Enqueues the reduction of the current state on the
+ * simulation's stream and writes the record to `out_device` (caller-owned
+ * device memory), without waiting and without allocating: its scratch (the
+ * partial records) is a static device array of the library, one 49 KiB copy
+ * per GPU, large enough for every grid (the reduction's first pass never
+ * exceeds 1024 workgroups).  The record equals smoke_get_step_norms's bit for
+ * bit; it never reuses an in-step result.  Read it after smoke_synchronize
+ * (a non-finite field shows as nonfinite > 0; no status).  All calls on one
+ * GPU share the scratch: calls that overlap on one GPU (simulations on
+ * different streams) race, so serialise them per GPU.  SMOKE_ERR_STATE for a
+ * simulation in a batch. */
+smoke_status smoke_get_step_norms_noalloc(smoke_sim* sim, smoke_step_norms* out_device);
 
 /* --------------------------------------------------------------- batches */
 /* Many independent simulations of different sizes advanced together: one

@@ -22,6 +22,7 @@
 //     (-u0 + u1 + -v0 + v1 + -w0 + w1, smokeSimulation.cu:379-381), and the
 //     right-hand side d of the projection (the divergence before it).
 #include <algorithm>
+#include <mutex>
 
 #include "step_device.h"
 
@@ -207,6 +208,42 @@ hipError_t launch_norms(const Grid& g, const T* rho, const T* u, const T* v, con
 }  // namespace
 
 size_t norms_scratch_bytes() { return (size_t)(kNormsBlocks + 1) * sizeof(NormAcc); }
+
+// Scratch of the allocation-free entry point (smoke_get_step_norms_noalloc):
+// the partial records and the reduced record, one copy per GPU the library
+// runs on, held until the process ends (49 KiB).  Every grid fits, because
+// the first pass never launches more than kNormsBlocks workgroups.  All calls
+// on one GPU share it: calls that overlap on one GPU race.
+static __device__ __align__(256) NormAcc scratch[kNormsBlocks + 1];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+hipError_t noalloc_norms_scratch(int device, void** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // hipGetSymbolAddress resolves the copy of the current device.
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+size_t noalloc_norms_scratch_bytes() { return sizeof(scratch); }
 
 
 hipError_t step_norms(const Grid& g, const float* rho, const float* u, const float* v,

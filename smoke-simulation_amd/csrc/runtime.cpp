@@ -1356,6 +1356,22 @@ smoke_status smoke_get_step_norms(smoke_sim* s, smoke_step_norms* out)
     return SMOKE_OK;
 }
 
+smoke_status smoke_get_step_norms_noalloc(smoke_sim* s, smoke_step_norms* out_device)
+{
+    if (!s || !out_device) return fail(SMOKE_ERR_INVALID, "get_step_norms_noalloc: NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->batch)
+        return fail(SMOKE_ERR_STATE, "get_step_norms_noalloc: the simulation belongs to a batch");
+    if (smk::noalloc_norms_scratch_bytes() < smk::norms_scratch_bytes())
+        return fail(SMOKE_ERR_INVALID, "get_step_norms_noalloc: the reduction needs %zu bytes of "
+                    "scratch, the static scratch holds %zu", smk::norms_scratch_bytes(),
+                    smk::noalloc_norms_scratch_bytes());
+    void* scratch = nullptr;
+    HIP_TRY(smk::noalloc_norms_scratch(s->device, &scratch));
+    HIP_TRY(smk::enqueue_norms(s, scratch, out_device));
+    return SMOKE_OK;
+}
+
 smoke_status smoke_timing_enable(smoke_sim* s, int enable)
 {
     if (!s) return fail(SMOKE_ERR_INVALID, "timing_enable: NULL sim");

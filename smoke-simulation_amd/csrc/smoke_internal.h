@@ -311,6 +311,10 @@ hipError_t step_norms(const Grid& g, const bf16s* rho, const bf16s* u, const bf1
                       const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
                       const float* d, int rhs, const uint8_t* flags, void* scratch,
                       smoke_step_norms* out, float dt, long long step, hipStream_t st);
+// Device address (on `device`) of the library's static norms scratch, for
+// the allocation-free entry point; resolved once per device.
+hipError_t noalloc_norms_scratch(int device, void** out);
+size_t noalloc_norms_scratch_bytes();
 
 // ------------------------------------------- mixed-precision pressure solve
 // Device buffers of the refinement loop (refine_kernels.hip), padded layout.

@@ -154,6 +154,7 @@ def _declare(lib):
         "smoke_timing_sweeps": (st, [vp, dp, llp]),
         "smoke_norms_enable": (st, [vp, i]),
         "smoke_get_step_norms": (st, [vp, ctypes.POINTER(StepNorms)]),
+        "smoke_get_step_norms_noalloc": (st, [vp, vp]),
         "smoke_batch_create": (st, [ctypes.POINTER(vp), i, fp, ctypes.POINTER(vp)]),
         "smoke_batch_destroy": (None, [vp]),
         "smoke_batch_step": (st, [vp, i]),

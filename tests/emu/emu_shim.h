@@ -16,6 +16,13 @@
 #define __shared__ static  // one block runs at a time, so a static is per-block LDS
 #undef __launch_bounds__
 #define __launch_bounds__(...)
+#ifndef __align__
+#define __align__(n) __attribute__((aligned(n)))
+#endif
+// A __device__ variable is a host global here; its "device address" is its
+// host address (emu_runtime.cpp's hipGetSymbolAddress).
+#undef HIP_SYMBOL
+#define HIP_SYMBOL(x) x
 
 struct EmuIdx {
     unsigned x, y, z;
