@@ -31,6 +31,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "smoke-simulation_amd"))
 
@@ -70,21 +72,25 @@ BYTES_REF_SWEEP = 7            # FP16 sweep: e, qi (2 B each), flags (1 B) in; e
 BYTES_REF_RESID = 19           # x, d (4), e (2), flags in; x, r out (4 each)
 BYTES_REF_NORM0 = 5            # d, flags in
 # BF16 fields: force 15, divergence 11, gradient 17, velocity advection 13,
-# density advection 11.
+# density advection 11.  FP32 density with BF16 velocity: force 17 (density
+# 4 B), divergence 11, gradient 17, velocity advection 13, density advection
+# 15 (density 4 B in and out).
 BYTES_FIXED_BF16 = 15 + 11 + 17 + 13 + 11
+BYTES_FIXED_BF16_VELOCITY = 17 + 11 + 17 + 13 + 15
+FIXED_BY_STORAGE = {0: BYTES_FIXED, 1: BYTES_FIXED_BF16, 2: BYTES_FIXED_BF16_VELOCITY}
 
 
 def step_bytes_per_cell(params, storage=0):
     """Algorithmic bytes per cell of one full step for the chosen projection
-    and field storage (0 FP32, 1 BF16)."""
+    and field storage (0 FP32, 1 BF16, 2 FP32 density with BF16 velocity)."""
     k = params["iterations"]
+    fixed = FIXED_BY_STORAGE[storage]
     if params["projection"] == 3:
         inner = params.get("inner_sweeps", 20)
         refinements = -(-k // inner)
-        fixed = BYTES_FIXED_BF16 if storage == 1 else BYTES_FIXED
         return fixed + BYTES_REF_NORM0 + refinements * (inner * BYTES_REF_SWEEP + BYTES_REF_RESID)
-    if storage == 1:
-        return BYTES_FIXED_BF16 + BYTES_SWEEP * (k if params["projection"] == 2 else 2 * k)
+    if storage != 0:
+        return fixed + BYTES_SWEEP * (k if params["projection"] == 2 else 2 * k)
     if params["projection"] == 0:
         # In-place red-black half-sweep on faces: every u, v, w face is read
         # and written once per half-sweep, plus the flags byte.
@@ -203,10 +209,11 @@ def latency(ms):
             "p99": round(float(np.percentile(ms, 99)), 4)}
 
 
-# The mixed-precision form: field storage, projection and its label.
+# The mixed-precision form: field storage (FP32 density, BF16 velocity),
+# projection (FP32 refinement around FP16 inner sweeps) and its label.
 MIXED = dict(projection=3, inner_sweeps=20)
-MIXED_STORAGE = 1
-MIXED_DTYPE = "bf16 fields, f16 inner sweeps, f32 math"
+MIXED_STORAGE = 2
+MIXED_DTYPE = "bf16 velocity, f32 density, f16 inner sweeps, f32 math"
 
 
 def error_check(sc, steps, mixed_iterations):
@@ -224,12 +231,14 @@ def error_check(sc, steps, mixed_iterations):
     sc = dict(sc, params={k: v for k, v in sc["params"].items()
                           if k not in ("inner_sweeps", "tolerance")})
     sc["params"]["projection"] = 2
-    # The mixed form and its two halves: BF16 fields alone (FP32 Jacobi) and
-    # the FP16-inner refined solve alone (FP32 fields).
+    # The mixed form (FP32 density, BF16 velocity, refined solve), the same
+    # with BF16 density too, and its two halves: BF16 velocity alone (FP32
+    # Jacobi) and the FP16-inner refined solve alone (FP32 fields).
     refined = dict(MIXED, iterations=mixed_iterations)
     forms = {"fp32": (smokesim.STORAGE_FP32, {}),
-             "mixed": (smokesim.STORAGE_BF16, refined),
-             "bf16_fields_fp32_jacobi": (smokesim.STORAGE_BF16, {}),
+             "mixed": (MIXED_STORAGE, refined),
+             "bf16_fields_refined": (smokesim.STORAGE_BF16, refined),
+             "bf16_velocity_fp32_jacobi": (smokesim.STORAGE_BF16_VELOCITY, {}),
              "fp32_fields_refined": (smokesim.STORAGE_FP32, refined)}
     got = {}
     for name, (storage, extra) in forms.items():
@@ -516,6 +525,26 @@ def run_batch(args, scs):
     ms_per_step = 1000.0 * wall / args.steps
     bpc = step_bytes_per_cell(scs[0]["params"])
     step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
+    # Dominant kernel: the persistent fused-sweep launch over every member,
+    # timed with HIP events on the batch's stream; compulsory bytes 13 B per
+    # interior cell of all members per launch.
+    launch_ms, launches = b.time_sweeps(reps=3)
+    plan = b.get_jacobi_plan()
+    comp = BYTES_SWEEP * b.interior_cells
+    achieved = comp / (launch_ms * 1e-3) / 1e9
+    sweeps_per_launch = k / launches
+    roofline = {"bound": "hbm", "kernel": "kb_jacobi", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_us": round(launch_ms * 1e3, 2),
+                "alg_bytes_per_launch": int(comp),
+                "alg_bytes_model": "compulsory: 13 B x interior cells of every member per launch",
+                "sweeps_per_launch": round(sweeps_per_launch, 3),
+                "effective": {"bytes_per_launch": int(comp * sweeps_per_launch),
+                              "frac": round(achieved * sweeps_per_launch / HBM_PEAK_GBS, 4),
+                              "model": "unfused: 13 B x interior cells x sweeps per launch"},
+                "jacobi_plan": plan}
+    comp_bpc = BYTES_FIXED + BYTES_SWEEP * launches
+    step_comp = cells * comp_bpc / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(cells * args.steps / wall, 1),
@@ -528,15 +557,53 @@ def run_batch(args, scs):
                                "one launch per stage, HIP graph",
                    "cells": cells, "simulations": len(scs), "parallelism": "single-gpu"},
         "latency_ms": latency(step_ms),
-        "step_roofline": {"bytes_per_cell": bpc, "achieved": round(step_eff, 1),
+        "step_roofline": {"bytes_per_cell": round(comp_bpc, 2), "achieved": round(step_comp, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(step_eff / HBM_PEAK_GBS, 4)},
-        "roofline": None, "autotune": tuned, "cpu_baseline": None,
+                          "frac": round(step_comp / HBM_PEAK_GBS, 4),
+                          "model": "compulsory bytes of the step's launches",
+                          "effective": {"bytes_per_cell": bpc, "achieved": round(step_eff, 1),
+                                        "frac": round(step_eff / HBM_PEAK_GBS, 4),
+                                        "model": "unfused streaming model (121 + 13 K B/cell)"}},
+        "roofline": roofline, "autotune": tuned, "cpu_baseline": None,
     }
     b.close()
     for s in sims:
         s.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle over the same members) ...")
+        out["cpu_baseline"] = cpu_baseline_batch(scs, args.cpu_budget)
     return out
+
+
+def cpu_baseline_batch(scs, budget_s):
+    """The oracle's FP32 step over the same batch members, every member one
+    step per batch step, on this host's cores, for a bounded number of batch
+    steps (at least one)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from smokesim import scenes
+    info = host_info()
+    threads = max(1, min(16, info["usable_cores"] or 1))
+    orc.set_threads(threads)
+    oracles = []
+    for sc in scs:
+        o = orc.Oracle(*sc["dims"], 32)
+        scenes.apply(o, sc)
+        oracles.append(o)
+    cells = sum(int(np.prod(sc["dims"])) for sc in scs)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        for o, sc in zip(oracles, scs):
+            o.step(sc["dt"], 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} batch step(s) of all {len(scs)} members ({cells} cells), oracle FP32 "
+                      f"restatement, members stepped one after another, OpenMP over z on "
+                      f"{threads} threads ({el:.1f} s)",
+            "host": info}
 
 
 def run_serve(args):

@@ -57,11 +57,15 @@ typedef enum smoke_projection {
 
 /* Storage of density and u, v, w (chosen at creation).  BF16 keeps FP32
  * arithmetic (widen on load, round to nearest even on store); pressure and
- * divergence stay FP32.  BF16 simulations use the pressure forms (RBSOR,
- * JACOBI, REFINED), not REFERENCE, and do not join batches. */
+ * divergence stay FP32.  BF16_VELOCITY stores u, v, w in BF16 and the density
+ * in FP32 (the density is re-rounded every step by the advection that
+ * carries it, velocity errors are damped by the projection).  Simulations
+ * with BF16 velocity use the pressure forms (RBSOR, JACOBI, REFINED), not
+ * REFERENCE, and do not join batches. */
 typedef enum smoke_storage {
     SMOKE_STORAGE_FP32 = 0,
-    SMOKE_STORAGE_BF16 = 1
+    SMOKE_STORAGE_BF16 = 1,
+    SMOKE_STORAGE_BF16_VELOCITY = 2
 } smoke_storage;
 
 /* How several obstacles combine into the interior mask.  LAST_WINS is the
@@ -384,6 +388,15 @@ smoke_status smoke_batch_autotune(smoke_batch* batch, smoke_jacobi_plan* chosen,
 smoke_status smoke_batch_step_ms(smoke_batch* batch, double* ms, int cap, int* count);
 /* Cells of all simulations of the batch (for cell-updates/s). */
 long long smoke_batch_cells(const smoke_batch* batch);
+/* Interior cells (n-2 per axis) of all simulations: the sweeps' domain. */
+long long smoke_batch_interior_cells(const smoke_batch* batch);
+/* Times the batch's pressure-sweep sequence with its current plan (HIP
+ * events on the batch's stream, one untimed sequence first, then `reps`):
+ * average time per sweep launch and launches per sequence.  The sweeps run on
+ * the last step's right-hand side; each step starts its solve from zero, so
+ * this changes no later result. */
+smoke_status smoke_batch_time_sweeps(smoke_batch* batch, int reps, double* ms_per_launch,
+                                     int* launches);
 
 /* --------------------------------------------------------------- serving */
 /* A request loop over a stream of independent simulation requests

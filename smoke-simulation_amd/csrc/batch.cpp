@@ -534,6 +534,38 @@ smoke_status smoke_batch_autotune(smoke_batch* b, smoke_jacobi_plan* chosen, dou
     return SMOKE_OK;
 }
 
+smoke_status smoke_batch_time_sweeps(smoke_batch* b, int reps, double* ms_per_launch, int* launches)
+{
+    if (!b || reps < 1) return fail(SMOKE_ERR_INVALID, "batch_time_sweeps: NULL batch or reps < 1");
+    if (b->broken) return fail(SMOKE_ERR_STATE, "batch_time_sweeps: a simulation was destroyed");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    int L = 0;
+    if (e == hipSuccess) e = enqueue_sweeps(b, b->plan, &L);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(ev[0], b->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = enqueue_sweeps(b, b->plan, &L);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return fail(SMOKE_ERR_HIP, "batch_time_sweeps: %s", hipGetErrorString(e));
+    if (ms_per_launch) *ms_per_launch = ms / (double)(reps * L);
+    if (launches) *launches = L;
+    return SMOKE_OK;
+}
+
+long long smoke_batch_interior_cells(const smoke_batch* b)
+{
+    if (!b) return 0;
+    long long n = 0;
+    for (const smk::SimDesc& d : b->h_sims) n += (long long)(d.g.nx - 2) * (d.g.ny - 2) * (d.g.nz - 2);
+    return n;
+}
+
 smoke_status smoke_batch_step_ms(smoke_batch* b, double* ms, int cap, int* count)
 {
     if (!b || (!ms && cap > 0)) return fail(SMOKE_ERR_INVALID, "batch_step_ms: NULL");

@@ -101,9 +101,9 @@ __device__ __forceinline__ void count_bad(NormAcc& a, float v)
     a.nonfinite += __builtin_isfinite(v) ? 0ull : 1ull;
 }
 
-template <typename T>
+template <typename T, typename TR>
 __global__ __launch_bounds__(BX * BY) void k_norms_part(
-    Grid g, const T* __restrict__ rho, const T* __restrict__ u, const T* __restrict__ v,
+    Grid g, const TR* __restrict__ rho, const T* __restrict__ u, const T* __restrict__ v,
     const T* __restrict__ w, const T* __restrict__ un, const T* __restrict__ vn,
     const T* __restrict__ wn, const float* __restrict__ d, int rhs, const uint8_t* __restrict__ flags,
     NormAcc* __restrict__ part)
@@ -184,8 +184,8 @@ __global__ void k_norms_publish(const NormAcc* __restrict__ r, smoke_step_norms*
     *out = o;
 }
 
-template <typename T>
-hipError_t launch_norms(const Grid& g, const T* rho, const T* u, const T* v, const T* w,
+template <typename T, typename TR>
+hipError_t launch_norms(const Grid& g, const TR* rho, const T* u, const T* v, const T* w,
                         const T* un, const T* vn, const T* wn, const float* d, int rhs,
                         const uint8_t* flags, void* scratch, smoke_step_norms* out, float dt,
                         long long step, hipStream_t st)
@@ -194,7 +194,7 @@ hipError_t launch_norms(const Grid& g, const T* rho, const T* u, const T* v, con
     const int blocks = std::min(kNormsBlocks, (rows + BY - 1) / BY);
     NormAcc* part = static_cast<NormAcc*>(scratch);
     NormAcc* res = part + kNormsBlocks;
-    hipLaunchKernelGGL(k_norms_part<T>, dim3((unsigned)blocks), dim3(BX, BY), 0, st, g, rho, u, v,
+    hipLaunchKernelGGL((k_norms_part<T, TR>), dim3((unsigned)blocks), dim3(BX, BY), 0, st, g, rho, u, v,
                        w, un, vn, wn, d, rhs, flags, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -245,7 +245,6 @@ hipError_t noalloc_norms_scratch(int device, void** out)
 
 size_t noalloc_norms_scratch_bytes() { return sizeof(scratch); }
 
-
 hipError_t step_norms(const Grid& g, const float* rho, const float* u, const float* v,
                       const float* w, const float* un, const float* vn, const float* wn,
                       const float* d, int rhs, const uint8_t* flags, void* scratch,
@@ -254,6 +253,13 @@ hipError_t step_norms(const Grid& g, const float* rho, const float* u, const flo
     return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step, st);
 }
 hipError_t step_norms(const Grid& g, const bf16s* rho, const bf16s* u, const bf16s* v,
+                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
+                      const float* d, int rhs, const uint8_t* flags, void* scratch,
+                      smoke_step_norms* out, float dt, long long step, hipStream_t st)
+{
+    return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step, st);
+}
+hipError_t step_norms(const Grid& g, const float* rho, const bf16s* u, const bf16s* v,
                       const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
                       const float* d, int rhs, const uint8_t* flags, void* scratch,
                       smoke_step_norms* out, float dt, long long step, hipStream_t st)

@@ -3,11 +3,14 @@
 //
 // smoke_proxy_geometry_device enqueues the kernel on the caller's stream into
 // the caller's device arrays.  smoke_proxy_geometry serves host arrays: it
-// keeps, per device, one workspace holding the largest view asked for so far
-// and one non-blocking stream, runs the kernel there and copies back exactly
-// the vertices and indices produced.
+// keeps, per device, one non-blocking stream and one pinned host workspace
+// holding the largest view asked for so far; the kernel writes its arrays
+// straight into that pinned memory (a few KB per view), so a view costs one
+// launch and one wait, then a host copy of exactly the vertices and indices
+// produced into the caller's arrays.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -20,8 +23,8 @@ namespace {
 struct ProxyWorkspace {
     int device = -1;
     hipStream_t stream = nullptr;
-    char* mem = nullptr;
-    int steps = 0;  // capacity in planes
+    char* mem = nullptr;  // pinned host memory the kernel writes
+    int steps = 0;        // capacity in planes
 };
 
 std::mutex g_proxy_mu;
@@ -62,12 +65,12 @@ smoke_status workspace(int steps, ProxyWorkspace** out)
         HIP_TRY(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
     }
     if (ws->steps < steps) {
-        if (ws->mem) HIP_TRY(hipFree(ws->mem));
+        if (ws->mem) HIP_TRY(hipHostFree(ws->mem));
         ws->mem = nullptr;
         ws->steps = 0;
         const size_t bytes = align256(vert_bytes(steps)) + align256(index_bytes(steps)) +
                              align256((size_t)steps * sizeof(smoke_proxy_plane)) + 256;
-        HIP_TRY(hipMalloc((void**)&ws->mem, bytes));
+        HIP_TRY(hipHostMalloc((void**)&ws->mem, bytes, hipHostMallocDefault));
         ws->steps = steps;
     }
     *out = ws;
@@ -113,16 +116,11 @@ smoke_status smoke_proxy_geometry(const float box_position[3], const float box_d
     p += align256((size_t)ws->steps * sizeof(smoke_proxy_plane));
     int* dc = reinterpret_cast<int*>(p);
     HIP_TRY(smk::proxy_geometry(v, steps, dv, di, dp, dc, ws->stream));
-    int counts[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(counts, dc, sizeof counts, hipMemcpyDeviceToHost, ws->stream));
     HIP_TRY(hipStreamSynchronize(ws->stream));
-    HIP_TRY(hipMemcpyAsync(vertices, dv, (size_t)counts[0] * 3 * sizeof(float), hipMemcpyDeviceToHost,
-                           ws->stream));
-    HIP_TRY(hipMemcpyAsync(indices, di, (size_t)counts[1] * sizeof(int16_t), hipMemcpyDeviceToHost,
-                           ws->stream));
-    HIP_TRY(hipMemcpyAsync(planes, dp, (size_t)steps * sizeof(smoke_proxy_plane), hipMemcpyDeviceToHost,
-                           ws->stream));
-    HIP_TRY(hipStreamSynchronize(ws->stream));
+    const int counts[2] = {dc[0], dc[1]};
+    std::memcpy(vertices, dv, (size_t)counts[0] * 3 * sizeof(float));
+    std::memcpy(indices, di, (size_t)counts[1] * sizeof(int16_t));
+    std::memcpy(planes, dp, (size_t)steps * sizeof(smoke_proxy_plane));
     *n_vertices = counts[0];
     *n_indices = counts[1];
     return SMOKE_OK;

@@ -62,7 +62,14 @@ constexpr int kStoredFields = 8;  // rho0 rho1 u0 u1 v0 v1 w0 w1 (FP32 or BF16)
 constexpr int kFloatFields = 3;   // P0 P1 d (always FP32)
 constexpr int kByteFields = 3;    // base s flags
 
-size_t stored_bytes(int storage) { return storage == SMOKE_STORAGE_BF16 ? 2 : 4; }
+bool storage_ok(int storage)
+{
+    return storage == SMOKE_STORAGE_FP32 || storage == SMOKE_STORAGE_BF16 ||
+           storage == SMOKE_STORAGE_BF16_VELOCITY;
+}
+// Bytes per element of the stored density and velocity fields.
+size_t density_bytes(int storage) { return storage == SMOKE_STORAGE_BF16 ? 2 : 4; }
+size_t velocity_bytes(int storage) { return storage == SMOKE_STORAGE_FP32 ? 4 : 2; }
 
 // Arena: stored fields, P0 P1 d, staging (reference-layout faces, FP32), then
 // the byte fields.
@@ -70,7 +77,7 @@ size_t arena_bytes(const Grid& g, int storage)
 {
     size_t e = (size_t)g.elems;
     size_t staging = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) * sizeof(float);
-    return kStoredFields * align256(e * stored_bytes(storage)) +
+    return 2 * align256(e * density_bytes(storage)) + 6 * align256(e * velocity_bytes(storage)) +
            kFloatFields * align256(e * sizeof(float)) + align256(staging) +
            kByteFields * align256(e);
 }
@@ -661,8 +668,7 @@ size_t smoke_footprint_bytes(unsigned nx, unsigned ny, unsigned nz)
 
 size_t smoke_footprint_bytes_ex(unsigned nx, unsigned ny, unsigned nz, int storage, int refined)
 {
-    if (!dims_ok(nx, ny, nz) || (storage != SMOKE_STORAGE_FP32 && storage != SMOKE_STORAGE_BF16))
-        return 0;
+    if (!dims_ok(nx, ny, nz) || !storage_ok(storage)) return 0;
     const Grid g = make_grid(nx, ny, nz);
     return arena_bytes(g, storage) + (refined ? refine_bytes(g) : 0);
 }
@@ -683,14 +689,13 @@ smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float*
     if (!dims_ok(nx, ny, nz))
         return fail(SMOKE_ERR_INVALID, "create: dims %ux%ux%u invalid (each >= 3, padded size < 2^31)",
                     nx, ny, nz);
-    if (storage != SMOKE_STORAGE_FP32 && storage != SMOKE_STORAGE_BF16)
-        return fail(SMOKE_ERR_INVALID, "create: unknown storage %d", storage);
+    if (!storage_ok(storage)) return fail(SMOKE_ERR_INVALID, "create: unknown storage %d", storage);
     smoke_sim* s = new smoke_sim();
     s->g = make_grid(nx, ny, nz);
     s->storage = storage;
     smoke_params_default(&s->prm);
-    // BF16 fields cannot run the face-based reference projection.
-    if (storage == SMOKE_STORAGE_BF16) s->prm.projection = SMOKE_PROJ_JACOBI;
+    // BF16 velocity cannot run the face-based reference projection.
+    if (storage != SMOKE_STORAGE_FP32) s->prm.projection = SMOKE_PROJ_JACOBI;
     auto bail = [&](smoke_status st) {
         smoke_destroy(s);
         return st;
@@ -706,19 +711,19 @@ smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float*
         return bail(fail(SMOKE_ERR_NOMEM, "create: cannot allocate %.2f GiB: %s", bytes / 1073741824.0,
                          hipGetErrorString(e)));
     }
-    const size_t sbytes = align256((size_t)s->g.elems * stored_bytes(storage));
     const size_t fbytes = align256((size_t)s->g.elems * sizeof(float));
     char* p = s->arena;
     for (int k = 0; k < kStoredFields; ++k) {
-        const int h = k & 1;
-        if (storage == SMOKE_STORAGE_BF16) {
+        const int h = k & 1, field = k >> 1;  // rho, u, v, w
+        const bool half = field == 0 ? density_bytes(storage) == 2 : velocity_bytes(storage) == 2;
+        if (half) {
             smk::bf16s** f[4] = {s->hrho, s->hu, s->hv, s->hw};
-            f[k >> 1][h] = reinterpret_cast<smk::bf16s*>(p);
+            f[field][h] = reinterpret_cast<smk::bf16s*>(p);
         } else {
             float** f[4] = {s->rho, s->u, s->v, s->w};
-            f[k >> 1][h] = reinterpret_cast<float*>(p);
+            f[field][h] = reinterpret_cast<float*>(p);
         }
-        p += sbytes;
+        p += align256((size_t)s->g.elems * (half ? 2 : 4));
     }
     float** fields[kFloatFields] = {&s->P[0], &s->P[1], &s->d};
     for (int k = 0; k < kFloatFields; ++k) {
@@ -871,7 +876,7 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
     if (st != SMOKE_OK) return st;
     std::lock_guard<std::mutex> lk(s->mu);
     SMOKE_REFUSE_IN_BATCH(s, "set_params");
-    if (s->storage == SMOKE_STORAGE_BF16 && p->projection == SMOKE_PROJ_REFERENCE)
+    if (s->storage != SMOKE_STORAGE_FP32 && p->projection == SMOKE_PROJ_REFERENCE)
         return fail(SMOKE_ERR_INVALID,
                     "set_params: BF16 storage runs the pressure forms (RBSOR, JACOBI, REFINED), not REFERENCE");
     if (p->obstacle_mode != s->prm.obstacle_mode) s->mask_dirty = true;

@@ -45,7 +45,8 @@ struct smoke_sim {
     float* P[2] = {nullptr, nullptr};
     float* d = nullptr;
     // SMOKE_STORAGE_BF16: density and u, v, w live in these instead (the
-    // float pointers above are then NULL).
+    // float pointers above are then NULL); SMOKE_STORAGE_BF16_VELOCITY: u, v,
+    // w live here and the density in rho.
     int storage = SMOKE_STORAGE_FP32;
     smk::bf16s* hrho[2] = {nullptr, nullptr};
     smk::bf16s* hu[2] = {nullptr, nullptr};
@@ -122,11 +123,12 @@ struct smoke_sim {
 
 namespace smk {
 // Calls fn(rho, u, v, w) with the simulation's field pointer pairs typed by
-// its storage (float* [2] or bf16s* [2]).
+// its storage (float* [2] or bf16s* [2], density and velocity each).
 template <typename Fn>
 auto with_fields(smoke_sim* s, Fn&& fn)
 {
     if (s->storage == SMOKE_STORAGE_BF16) return fn(s->hrho, s->hu, s->hv, s->hw);
+    if (s->storage == SMOKE_STORAGE_BF16_VELOCITY) return fn(s->rho, s->hu, s->hv, s->hw);
     return fn(s->rho, s->u, s->v, s->w);
 }
 // Enqueues the step-norms reduction of the simulation's current state on its

@@ -284,6 +284,16 @@ hipError_t advect_velocity(const Grid& g, const bf16s* u0, const bf16s* v0, cons
 hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s* u,
                           const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
                           hipStream_t st);
+// FP32 density with BF16 velocity (the split storage of the mixed form).
+hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* rho,
+                 const uint8_t* flags, const ForceArgs& a, hipStream_t st);
+hipError_t advect_density(const Grid& g, const float* r0, float* r1, const bf16s* u,
+                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
+                          hipStream_t st);
+hipError_t step_norms(const Grid& g, const float* rho, const bf16s* u, const bf16s* v,
+                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
+                      const float* d, int rhs, const uint8_t* flags, void* scratch,
+                      smoke_step_norms* out, float dt, long long step, hipStream_t st);
 
 // ------------------------------------------------------------ step norms
 // Result record of the norms reduction (norms_kernels.hip), read back by the

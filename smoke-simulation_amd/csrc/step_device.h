@@ -91,10 +91,10 @@ struct ForceIn {
     uint8_t fl;
     bool ok;
 };
-template <typename T>
+template <typename T, typename TR>
 __device__ __forceinline__ ForceIn force_load(const Grid& g, const T* __restrict__ u,
                                               const T* __restrict__ v, const T* __restrict__ w,
-                                              const T* __restrict__ rho,
+                                              const TR* __restrict__ rho,
                                               const uint8_t* __restrict__ flags, int x, int y, int z)
 {
     ForceIn in{0.0f, 0.0f, 0.0f, 0.0f, 0, false};
@@ -136,9 +136,9 @@ __device__ __forceinline__ void force_apply(const Grid& g, T* __restrict__ u, T*
     }
     if (write_v) stf(v, i, vv);
 }
-template <int NZ, typename T>
+template <int NZ, typename T, typename TR>
 __device__ __forceinline__ void force_col(const Grid& g, T* __restrict__ u, T* __restrict__ v,
-                                          T* __restrict__ w, const T* __restrict__ rho,
+                                          T* __restrict__ w, const TR* __restrict__ rho,
                                           const uint8_t* __restrict__ flags, const ForceArgs& a,
                                           int x, int y, int z0)
 {
@@ -148,9 +148,9 @@ __device__ __forceinline__ void force_col(const Grid& g, T* __restrict__ u, T* _
 #pragma unroll
     for (int j = 0; j < NZ; ++j) force_apply(g, u, v, w, a, in[j], x, y, z0 + j);
 }
-template <typename T>
+template <typename T, typename TR>
 __device__ __forceinline__ void force_at(const Grid& g, T* __restrict__ u, T* __restrict__ v,
-                                         T* __restrict__ w, const T* __restrict__ rho,
+                                         T* __restrict__ w, const TR* __restrict__ rho,
                                          const uint8_t* __restrict__ flags, const ForceArgs& a,
                                          int x, int y, int z)
 {
@@ -858,11 +858,11 @@ __device__ __forceinline__ void advect_velocity_march(const Grid& g, const T* __
 // unconditionally too (a NaN position clamps to a valid cell as well); the
 // column version issues the face loads of all NZ cells, then the corner loads
 // of all NZ samples, then stores.
-template <int NZ, typename T>
+template <int NZ, typename T, typename TV>
 __device__ __forceinline__ void advect_density_col(const Grid& g, const T* __restrict__ r0,
-                                                   T* __restrict__ r1, const T* __restrict__ u,
-                                                   const T* __restrict__ v,
-                                                   const T* __restrict__ w,
+                                                   T* __restrict__ r1, const TV* __restrict__ u,
+                                                   const TV* __restrict__ v,
+                                                   const TV* __restrict__ w,
                                                    const uint8_t* __restrict__ flags, float dt,
                                                    int x, int y, int z0)
 {
@@ -909,11 +909,11 @@ __device__ __forceinline__ void advect_density_col(const Grid& g, const T* __res
 // plane z is computed.  Samples whose corners lie in the ring read LDS, a
 // back-trace longer than one cell reads global memory.  Same values, same
 // arithmetic as advect_density_col: the results are identical.
-template <int AY, int ZC, typename T>
+template <int AY, int ZC, typename T, typename TV>
 __device__ __forceinline__ void advect_density_march(const Grid& g, const T* __restrict__ r0,
-                                                     T* __restrict__ r1, const T* __restrict__ u,
-                                                     const T* __restrict__ v,
-                                                     const T* __restrict__ w,
+                                                     T* __restrict__ r1, const TV* __restrict__ u,
+                                                     const TV* __restrict__ v,
+                                                     const TV* __restrict__ w,
                                                      const uint8_t* __restrict__ flags, float dt,
                                                      int x0, int y0, int z0,
                                                      AdvRing<BX, AY, 4, 1, 2>& ring)
@@ -978,10 +978,10 @@ __device__ __forceinline__ void advect_density_march(const Grid& g, const T* __r
     }
 }
 
-template <typename T>
+template <typename T, typename TV>
 __device__ __forceinline__ void advect_density_at(const Grid& g, const T* __restrict__ r0,
-                                                  T* __restrict__ r1, const T* __restrict__ u,
-                                                  const T* __restrict__ v, const T* __restrict__ w,
+                                                  T* __restrict__ r1, const TV* __restrict__ u,
+                                                  const TV* __restrict__ v, const TV* __restrict__ w,
                                                   const uint8_t* __restrict__ flags, float dt,
                                                   int x, int y, int z)
 {

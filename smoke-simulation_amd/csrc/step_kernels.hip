@@ -236,9 +236,9 @@ __global__ void k_source(Grid g, T* __restrict__ r0, T* __restrict__ r1, float4 
 
 // ---------------------------------------------------------- step stages
 
-template <typename T>
+template <typename T, typename TR>
 __global__ void k_force(Grid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
-                        const T* __restrict__ rho, const uint8_t* __restrict__ flags, ForceArgs a)
+                        const TR* __restrict__ rho, const uint8_t* __restrict__ flags, ForceArgs a)
 {
     dev::force_col<kZ>(g, u, v, w, rho, flags, a, blockIdx.x * BX + threadIdx.x,
                        blockIdx.y * BY + threadIdx.y + 1, blockIdx.z * kZ);
@@ -402,10 +402,10 @@ __global__ __launch_bounds__(BX * BY) void k_advect_velocity(
 
 // Density advection: a block of BX x BY threads marches a column of BX x BY
 // cells through kAdvZC planes (dev::advect_density_march), in XCD tile order.
-template <typename T>
+template <typename T, typename TV>
 __global__ __launch_bounds__(BX * BY) void k_advect_density(
-    Grid g, const T* __restrict__ r0, T* __restrict__ r1, const T* __restrict__ u,
-    const T* __restrict__ v, const T* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
+    Grid g, const T* __restrict__ r0, T* __restrict__ r1, const TV* __restrict__ u,
+    const TV* __restrict__ v, const TV* __restrict__ w, const uint8_t* __restrict__ flags, float dt)
 {
     __shared__ dev::AdvRing<BX, BY, 4, 1, 2> ring;
     const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
@@ -615,12 +615,12 @@ hipError_t fill_source(const Grid& g, bf16s* rho0, bf16s* rho1, float4 sph, hipS
     return launch_source(g, rho0, rho1, sph, st, launched);
 }
 
-template <typename T>
-static hipError_t launch_force(const Grid& g, T* u, T* v, T* w, const T* rho, const uint8_t* flags,
+template <typename T, typename TR>
+static hipError_t launch_force(const Grid& g, T* u, T* v, T* w, const TR* rho, const uint8_t* flags,
                                const ForceArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_force<T>, grid_col(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u, v, w,
-                       rho, flags, a);
+    hipLaunchKernelGGL((k_force<T, TR>), grid_col(g.nx, g.ny - 1, g.nz), dim3(BX, BY), 0, st, g, u,
+                       v, w, rho, flags, a);
     return hipGetLastError();
 }
 hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
@@ -629,6 +629,11 @@ hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
     return launch_force(g, u, v, w, rho, flags, a, st);
 }
 hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const bf16s* rho,
+                 const uint8_t* flags, const ForceArgs& a, hipStream_t st)
+{
+    return launch_force(g, u, v, w, rho, flags, a, st);
+}
+hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* rho,
                  const uint8_t* flags, const ForceArgs& a, hipStream_t st)
 {
     return launch_force(g, u, v, w, rho, flags, a, st);
@@ -810,14 +815,14 @@ hipError_t advect_velocity(const Grid& g, const bf16s* u0, const bf16s* v0, cons
     return launch_advect_velocity(g, u0, v0, w0, u1, v1, w1, flags, dt, st);
 }
 
-template <typename T>
-static hipError_t launch_advect_density(const Grid& g, const T* r0, T* r1, const T* u, const T* v,
-                                        const T* w, const uint8_t* flags, float dt, hipStream_t st)
+template <typename T, typename TV>
+static hipError_t launch_advect_density(const Grid& g, const T* r0, T* r1, const TV* u, const TV* v,
+                                        const TV* w, const uint8_t* flags, float dt, hipStream_t st)
 {
     const dim3 grid((unsigned)((g.nx - 1 + BX - 1) / BX), (unsigned)((g.ny - 2 + BY - 1) / BY),
                     (unsigned)((g.nz - 2 + kAdvZC - 1) / kAdvZC));
-    hipLaunchKernelGGL(k_advect_density<T>, grid, dim3(BX, BY), 0, st, g, r0, r1, u, v, w, flags,
-                       dt);
+    hipLaunchKernelGGL((k_advect_density<T, TV>), grid, dim3(BX, BY), 0, st, g, r0, r1, u, v, w,
+                       flags, dt);
     return hipGetLastError();
 }
 hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float* u,
@@ -827,6 +832,12 @@ hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float
     return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
 }
 hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s* u,
+                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
+                          hipStream_t st)
+{
+    return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
+}
+hipError_t advect_density(const Grid& g, const float* r0, float* r1, const bf16s* u,
                           const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
                           hipStream_t st)
 {

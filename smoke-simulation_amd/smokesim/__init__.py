@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libsmokesim.so")
 SMOKE_OK = 0
 ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_NONFINITE = 1, 2, 3, 4, 5
 PROJ_REFERENCE, PROJ_RBSOR, PROJ_JACOBI, PROJ_REFINED = 0, 1, 2, 3
-STORAGE_FP32, STORAGE_BF16 = 0, 1
+STORAGE_FP32, STORAGE_BF16, STORAGE_BF16_VELOCITY = 0, 1, 2
 OBSTACLES_LAST_WINS, OBSTACLES_UNION = 0, 1
 STAGE_SCENE, STAGE_FORCE, STAGE_PROJECT, STAGE_ADVECT_VEL, STAGE_ADVECT_DENS = range(5)
 STAGE_COUNT = 5
@@ -164,6 +164,8 @@ def _declare(lib):
         "smoke_batch_autotune": (st, [vp, ctypes.POINTER(JacobiPlan), dp]),
         "smoke_batch_step_ms": (st, [vp, dp, i, ip]),
         "smoke_batch_cells": (ctypes.c_longlong, [vp]),
+        "smoke_batch_interior_cells": (ctypes.c_longlong, [vp]),
+        "smoke_batch_time_sweeps": (st, [vp, i, dp, ip]),
         "smoke_server_create": (st, [ctypes.c_size_t, i, ctypes.POINTER(vp)]),
         "smoke_server_destroy": (None, [vp]),
         "smoke_server_submit": (st, [vp, ctypes.POINTER(Request), llp]),
@@ -549,6 +551,16 @@ class Batch:
         ms = ctypes.c_double()
         _check(self.lib.smoke_batch_autotune(self.h, ctypes.byref(p), ctypes.byref(ms)))
         return p.as_dict(), ms.value
+
+    @property
+    def interior_cells(self):
+        return int(self.lib.smoke_batch_interior_cells(self.h))
+
+    def time_sweeps(self, reps=3):
+        """(ms per sweep launch, launches per sequence) of the batch's sweeps."""
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _check(self.lib.smoke_batch_time_sweeps(self.h, int(reps), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def step_ms(self, cap=1 << 16):
         buf = (ctypes.c_double * cap)()

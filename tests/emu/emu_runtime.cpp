@@ -19,7 +19,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <functional>
+#include <memory>
 #include <set>
 #include <thread>
 #include <vector>
@@ -56,6 +58,10 @@ namespace {
 
 std::set<void*> g_device;
 std::barrier<>* g_barrier = nullptr;
+// One barrier per wave of 64 threads for lane exchanges (DPP, shuffles): a
+// wave may skip work that other waves of the block do (wave-uniform
+// branches), so exchanges synchronise the wave, not the block.
+std::vector<std::barrier<>*> g_wave_barrier;
 bool g_threaded = false;
 
 [[noreturn]] void die(const char* what)
@@ -72,9 +78,10 @@ void check(hipEvent_t e)
     if (!e || e->magic != kEvent) die("use of an invalid event");
 }
 
-// Lane exchanges use two buffers in turn, so one barrier per exchange
-// suffices: a thread can only overwrite a buffer after every thread has
-// passed the next exchange's barrier, i.e. finished reading it.
+// Lane exchanges use two buffers in turn, so one wave barrier per exchange
+// suffices: a thread can only overwrite a buffer after every thread of its
+// wave (the only readers of its slot) has passed the next exchange's
+// barrier, i.e. finished reading it.
 unsigned long long g_xbuf[2][1024];
 thread_local unsigned g_xcount = 0;
 
@@ -155,6 +162,12 @@ void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 bloc
     }
     std::barrier<> bar((std::ptrdiff_t)nt);
     g_barrier = &bar;
+    std::vector<std::unique_ptr<std::barrier<>>> waves;
+    g_wave_barrier.clear();
+    for (unsigned w = 0; w * 64 < nt; ++w) {
+        waves.emplace_back(new std::barrier<>((std::ptrdiff_t)std::min(64u, nt - 64 * w)));
+        g_wave_barrier.push_back(waves.back().get());
+    }
     g_threaded = true;
     pool().run(nt, [&](unsigned t) {
         blockDim = bd;
@@ -186,7 +199,7 @@ static unsigned long long exchange(unsigned long long bits, int partner_t)
     const int t = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
     unsigned long long* buf = g_xbuf[g_xcount++ & 1];
     buf[t] = bits;
-    g_barrier->arrive_and_wait();
+    g_wave_barrier[t / 64]->arrive_and_wait();
     return buf[partner_t];
 }
 

@@ -218,3 +218,47 @@ def test_mixed_form_at_256_stays_close_to_fp32():
     assert nref == 5 and rel < 1.0
     a.close()
     b.close()
+
+
+def test_bf16_velocity_stages_round_only_the_velocity():
+    # The split storage (FP32 density, BF16 u, v, w): stage by stage over two
+    # steps, with the FP32 simulation re-fed the split one's fields before
+    # every stage, velocity outputs are the BF16 rounding of the FP32 stage's
+    # and density outputs equal the FP32 stage's, bit for bit.
+    sc = scenes.test_scene((66, 20, 18), seed=53, projection=smokesim.PROJ_JACOBI, iterations=6,
+                           noise=0.5)
+    g32, gs = gpu_for(sc), gpu_for(sc, storage=smokesim.STORAGE_BF16_VELOCITY)
+    names = [("rho", h) for h in (0, 1)] + [(c, h) for c in range(3) for h in (0, 1)]
+
+    def fields(g):
+        return {k: (g.get_density(k[1]) if k[0] == "rho" else g.get_velocity(k[0], k[1]))
+                for k in names}
+
+    for step in range(2):
+        for stage in range(smokesim.STAGE_COUNT):
+            for k, a in fields(gs).items():
+                if k[0] == "rho":
+                    g32.set_density(k[1], a)
+                else:
+                    g32.set_velocity(k[0], k[1], a)
+            g32.run_stage(stage, sc["dt"])
+            gs.run_stage(stage, sc["dt"])
+            f32, fs = fields(g32), fields(gs)
+            for k in names:
+                want = f32[k].view(np.uint32) if k[0] == "rho" else _rne_bf16_bits(f32[k])
+                assert np.array_equal(fs[k].view(np.uint32), want), (step, stage, k)
+            assert np.array_equal(gs.get_pressure().view(np.uint32),
+                                  g32.get_pressure().view(np.uint32)), (step, stage)
+    g32.close()
+    gs.close()
+
+
+@pytest.mark.parametrize("projection", [smokesim.PROJ_JACOBI, smokesim.PROJ_REFINED])
+@pytest.mark.parametrize("dims,seed", CASES[:2])
+def test_bf16_velocity_within_tolerance(dims, seed, projection):
+    g, o = run_pair(dims, seed, smokesim.STORAGE_BF16_VELOCITY, projection)
+    err = field_errors(g, o)
+    print("bf16-velocity errors", dims, projection, err)
+    assert err["rho"] <= 2e-3, err
+    assert max(err["u"], err["v"], err["w"], err["pressure"]) <= 2e-2, err
+    g.close()
