@@ -62,7 +62,8 @@ def describe(sc, storage=0):
             2: "Jacobi", 3: f"refined (FP32 outer, FP16 inner x{p.get('inner_sweeps')} per "
                             "refinement, FP64 norms), total sweeps"}[p["projection"]]
     solid = "floor + seeded boxes 5%" if sc.get("solid") is not None else "floor"
-    fields = {0: "FP32", 1: "BF16 fields, FP32 math", 2: "FP32 density, BF16 velocity, FP32 math"
+    fields = {0: "FP32", 1: "BF16 fields, FP32 math", 2: "FP32 density, BF16 velocity, FP32 math",
+              3: "FP16 fields, FP32 math", 4: "FP32 density, FP16 velocity, FP32 math"
               }.get(storage, "FP32")
     return (f"{sc['name']}: {nx}x{ny}x{nz} {fields}, {solid}, {len(sc['obstacles'])} obstacle(s), "
             f"{len(sc['sources'])} source(s), {proj} x{p['iterations']}, dt={sc['dt']}")
@@ -71,18 +72,20 @@ def describe(sc, storage=0):
 BYTES_REF_SWEEP = 7            # FP16 sweep: e, qi (2 B each), flags (1 B) in; e out (2 B)
 BYTES_REF_RESID = 19           # x, d (4), e (2), flags in; x, r out (4 each)
 BYTES_REF_NORM0 = 5            # d, flags in
-# BF16 fields: force 15, divergence 11, gradient 17, velocity advection 13,
-# density advection 11.  FP32 density with BF16 velocity: force 17 (density
-# 4 B), divergence 11, gradient 17, velocity advection 13, density advection
-# 15 (density 4 B in and out).
-BYTES_FIXED_BF16 = 15 + 11 + 17 + 13 + 11
-BYTES_FIXED_BF16_VELOCITY = 17 + 11 + 17 + 13 + 15
-FIXED_BY_STORAGE = {0: BYTES_FIXED, 1: BYTES_FIXED_BF16, 2: BYTES_FIXED_BF16_VELOCITY}
+# 16-bit fields (BF16 or FP16): force 15, divergence 11, gradient 17,
+# velocity advection 13, density advection 11.  FP32 density with 16-bit
+# velocity: force 17 (density 4 B), divergence 11, gradient 17, velocity
+# advection 13, density advection 15 (density 4 B in and out).
+BYTES_FIXED_HALF = 15 + 11 + 17 + 13 + 11
+BYTES_FIXED_HALF_VELOCITY = 17 + 11 + 17 + 13 + 15
+FIXED_BY_STORAGE = {0: BYTES_FIXED, 1: BYTES_FIXED_HALF, 2: BYTES_FIXED_HALF_VELOCITY,
+                    3: BYTES_FIXED_HALF, 4: BYTES_FIXED_HALF_VELOCITY}
 
 
 def step_bytes_per_cell(params, storage=0):
     """Algorithmic bytes per cell of one full step for the chosen projection
-    and field storage (0 FP32, 1 BF16, 2 FP32 density with BF16 velocity)."""
+    and field storage (0 FP32, 1 BF16, 2 FP32 density with BF16 velocity, 3
+    FP16, 4 FP32 density with FP16 velocity)."""
     k = params["iterations"]
     fixed = FIXED_BY_STORAGE[storage]
     if params["projection"] == 3:
@@ -209,17 +212,18 @@ def latency(ms):
             "p99": round(float(np.percentile(ms, 99)), 4)}
 
 
-# The mixed-precision form: field storage (FP32 density, BF16 velocity),
+# The mixed-precision form: field storage (density and velocity in FP16),
 # projection (FP32 refinement around FP16 inner sweeps) and its label.
 MIXED = dict(projection=3, inner_sweeps=20)
-MIXED_STORAGE = 2
-MIXED_DTYPE = "bf16 velocity, f32 density, f16 inner sweeps, f32 math"
+MIXED_STORAGE = 3
+MIXED_DTYPE = "f16 density and velocity, f16 inner sweeps, f32 math"
 
 
 def error_check(sc, steps, mixed_iterations):
     """Per-field relative L2 and max-abs error after `steps` steps of the FP32
-    form (Jacobi) and of the mixed form (BF16 fields, REFINED) against the
-    FP64 oracle (Jacobi, same sweep count), all from the same inputs."""
+    form (Jacobi), the mixed form (FP16 fields, REFINED) and the other 16-bit
+    storage forms against the FP64 oracle (Jacobi, same sweep count), all from
+    the same inputs."""
     import numpy as np
     import smokesim
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -231,15 +235,17 @@ def error_check(sc, steps, mixed_iterations):
     sc = dict(sc, params={k: v for k, v in sc["params"].items()
                           if k not in ("inner_sweeps", "tolerance")})
     sc["params"]["projection"] = 2
-    # The mixed form (FP32 density, BF16 velocity, refined solve), the same
-    # with BF16 density too, and its two halves: BF16 velocity alone (FP32
-    # Jacobi) and the FP16-inner refined solve alone (FP32 fields).
+    # The mixed form (FP16 fields, refined solve), its halves (FP16 fields
+    # with FP32 Jacobi; the FP16-inner refined solve with FP32 fields), FP16
+    # velocity with FP32 density, and the BF16 storage forms.
     refined = dict(MIXED, iterations=mixed_iterations)
     forms = {"fp32": (smokesim.STORAGE_FP32, {}),
              "mixed": (MIXED_STORAGE, refined),
+             "fp16_fields_fp32_jacobi": (smokesim.STORAGE_FP16, {}),
+             "fp32_fields_refined": (smokesim.STORAGE_FP32, refined),
+             "fp16_velocity_refined": (smokesim.STORAGE_FP16_VELOCITY, refined),
              "bf16_fields_refined": (smokesim.STORAGE_BF16, refined),
-             "bf16_velocity_fp32_jacobi": (smokesim.STORAGE_BF16_VELOCITY, {}),
-             "fp32_fields_refined": (smokesim.STORAGE_FP32, refined)}
+             "bf16_velocity_refined": (smokesim.STORAGE_BF16_VELOCITY, refined)}
     got = {}
     for name, (storage, extra) in forms.items():
         s = smokesim.Simulation(nx, ny, nz, storage=storage)
@@ -730,12 +736,28 @@ def run_proxy(args):
         hip.hipEventDestroy(e)
     for p in ptrs:
         hip.hipFree(p)
-    # Host entry point: the per-frame cost a renderer sees with host arrays.
+    # Host entry point: the per-frame cost a renderer sees with host arrays,
+    # called through ctypes with its arguments built once, so the figure is
+    # the library's (launch, wait, copies out), not Python's.
     verts, idx, tbl = smokesim.proxy_geometry(*view, planes)
+    fp = ctypes.POINTER(ctypes.c_float)
+    vecs = [np.ascontiguousarray(v, dtype=np.float32).reshape(3) for v in view]
+    hv = np.empty((planes * smokesim.PROXY_VERTICES_PER_PLANE, 3), np.float32)
+    hi = np.empty(planes * smokesim.PROXY_INDICES_PER_PLANE, np.int16)
+    ht = np.empty((planes, 2), np.int32)
+    nv, ni = ctypes.c_int(0), ctypes.c_int(0)
+    call = [v.ctypes.data_as(fp) for v in vecs] + [
+        planes, hv.ctypes.data_as(fp), hi.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+        ht.ctypes.data_as(ctypes.POINTER(smokesim.ProxyPlane)), ctypes.byref(nv), ctypes.byref(ni)]
+    fn = smokesim.load().smoke_proxy_geometry
+    for _ in range(args.warmup):
+        fn(*call)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        smokesim.proxy_geometry(*view, planes)
+        fn(*call)
     host_us = 1e6 * (time.perf_counter() - t0) / args.steps
+    if (nv.value, ni.value) != (len(verts), len(idx)) or not np.array_equal(hi[:ni.value], idx):
+        raise SystemExit("proxy geometry: timed calls differ from the checked call")
     out_bytes = verts.nbytes + idx.nbytes + tbl.nbytes
     cpu = None
     if not args.no_cpu_baseline:
@@ -745,13 +767,25 @@ def run_proxy(args):
         if not (np.array_equal(ri, idx) and np.array_equal(rp, tbl)
                 and np.array_equal(rv.view(np.uint32), verts.view(np.uint32))):
             raise SystemExit("proxy geometry differs from the oracle")
+        # The oracle's C function, timed the same way (arguments built once).
+        ov = np.empty((planes * orc.PROXY_MAX_VERTS_PER_PLANE, 3), np.float32)
+        oi = np.empty(planes * orc.PROXY_MAX_INDICES_PER_PLANE, np.int16)
+        ot = np.empty((planes, 2), np.int32)
+        ocall = [v.ctypes.data_as(fp) for v in vecs] + [
+            planes, ov.ctypes.data_as(fp), oi.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+            ot.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(ctypes.c_int(0)),
+            ctypes.byref(ctypes.c_int(0))]
+        ofn = orc.load().orc_proxy_geometry
         n, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < min(args.cpu_budget, 5.0):
-            orc.proxy_geometry(*view, planes)
-            n += 1
+            for _ in range(100):
+                ofn(*ocall)
+            n += 100
         cpu_s = (time.perf_counter() - t0) / n
         cpu = {"value": round(planes / cpu_s, 1), "unit": "planes/s", "cores": 1, "kind": "port",
-               "sample": f"{n} views of {planes} planes, sequential FP32 oracle"}
+               "sample": f"{n} views of {planes} planes, sequential FP32 oracle (C, called "
+                         "through ctypes with prebuilt arguments)",
+               "us_per_view": round(1e6 * cpu_s, 2)}
     achieved = out_bytes / (kernel_us * 1e-6) / 1e9
     return {
         "metric": "proxy planes/s", "value": round(planes / (kernel_us * 1e-6), 1),
@@ -793,7 +827,7 @@ def main():
     ap.add_argument("--reference-80", action="store_true",
                     help="also time config c0 (reference_80) on the GPU and the host cores")
     ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32",
-                    help="mixed: BF16 fields, REFINED projection (FP16 inner sweeps, 20 per "
+                    help="mixed: FP16 fields, REFINED projection (FP16 inner sweeps, 20 per "
                          "refinement, same total sweep count)")
     ap.add_argument("--error-steps", type=int, default=0,
                     help="also step the FP32 and mixed forms and the FP64 oracle this many steps "

@@ -55,17 +55,21 @@ typedef enum smoke_projection {
     SMOKE_PROJ_REFINED = 3
 } smoke_projection;
 
-/* Storage of density and u, v, w (chosen at creation).  BF16 keeps FP32
- * arithmetic (widen on load, round to nearest even on store); pressure and
- * divergence stay FP32.  BF16_VELOCITY stores u, v, w in BF16 and the density
- * in FP32 (the density is re-rounded every step by the advection that
- * carries it, velocity errors are damped by the projection).  Simulations
- * with BF16 velocity use the pressure forms (RBSOR, JACOBI, REFINED), not
+/* Storage of density and u, v, w (chosen at creation).  The 16-bit forms
+ * keep FP32 arithmetic (widen on load, round to nearest even on store);
+ * pressure and divergence stay FP32.  BF16 / FP16: density and velocity in
+ * BF16 (8 significant bits, FP32's range) or IEEE binary16 (11 significant
+ * bits, magnitudes up to 65504: the clamp bounds the speeds far below that;
+ * a value beyond it becomes Inf, which smoke_get_step_norms reports).
+ * *_VELOCITY: u, v, w in 16 bits and the density in FP32.  Simulations with
+ * 16-bit velocity use the pressure forms (RBSOR, JACOBI, REFINED), not
  * REFERENCE, and do not join batches. */
 typedef enum smoke_storage {
     SMOKE_STORAGE_FP32 = 0,
     SMOKE_STORAGE_BF16 = 1,
-    SMOKE_STORAGE_BF16_VELOCITY = 2
+    SMOKE_STORAGE_BF16_VELOCITY = 2,
+    SMOKE_STORAGE_FP16 = 3,
+    SMOKE_STORAGE_FP16_VELOCITY = 4
 } smoke_storage;
 
 /* How several obstacles combine into the interior mask.  LAST_WINS is the
@@ -346,9 +350,9 @@ smoke_status smoke_get_step_norms(smoke_sim* sim, smoke_step_norms* out);
  * reference counterpart.  Enqueues the reduction of the current state on the
  * simulation's stream and writes the record to `out_device` (caller-owned
  * device memory), without waiting and without allocating: its scratch (the
- * partial records) is a static device array of the library, one 49 KiB copy
+ * partial records) is a static device array of the library, one 192 KiB copy
  * per GPU, large enough for every grid (the reduction's first pass never
- * exceeds 1024 workgroups).  The record equals smoke_get_step_norms's bit for
+ * exceeds 4096 workgroups).  The record equals smoke_get_step_norms's bit for
  * bit; it never reuses an in-step result.  Read it after smoke_synchronize
  * (a non-finite field shows as nonfinite > 0; no status).  All calls on one
  * GPU share the scratch: calls that overlap on one GPU (simulations on

@@ -260,7 +260,7 @@ smoke_status smoke_batch_create(smoke_sim* const* sims, int count, const float* 
         const smoke_params& p = s->prm;
         const smoke_params& p0 = sims[0]->prm;
         if (s->storage != SMOKE_STORAGE_FP32)
-            return fail(SMOKE_ERR_INVALID, "batch_create: simulation %d stores BF16 fields (batches are FP32)", i);
+            return fail(SMOKE_ERR_INVALID, "batch_create: simulation %d stores 16-bit fields (batches are FP32)", i);
         if (p.obstacle_velocity)
             return fail(SMOKE_ERR_INVALID, "batch_create: simulation %d uses the moving-obstacle condition (batches do not)", i);
         if (p.vorticity != 0.0f)

@@ -4,7 +4,7 @@
 // density, and a count of non-finite values, reduced on the device in a fixed
 // order, so the result depends only on the fields and the grid dims.
 //
-// Pass 1 (k_norms_part): a 1-D grid of at most kNormsBlocks workgroups of
+// Pass 1 (k_norms_part): a 1-D grid of at most kNormsBlocks (4096) workgroups of
 // 64 x 4 threads walks the rows (y, z) of the cell range [0,nx) x [0,ny) x
 // [0,nz); lane x takes cells x, x + 64, ... of its row.  Each thread sums in
 // its own fixed order, each wave reduces its 64 lanes with a butterfly of
@@ -114,6 +114,7 @@ __global__ __launch_bounds__(BX * BY) void k_norms_part(
     const int rows = g.ny * g.nz;
     for (int r = (int)(blockIdx.x * BY + threadIdx.y); r < rows; r += (int)(gridDim.x * BY)) {
         const int y = r % g.ny, z = r / g.ny;
+#pragma unroll 2
         for (int x = (int)threadIdx.x; x < g.nx; x += BX) {
             const int i = g.idx(x, y, z);
             const float q = dev::ldf(rho, i);
@@ -211,7 +212,7 @@ size_t norms_scratch_bytes() { return (size_t)(kNormsBlocks + 1) * sizeof(NormAc
 
 // Scratch of the allocation-free entry point (smoke_get_step_norms_noalloc):
 // the partial records and the reduced record, one copy per GPU the library
-// runs on, held until the process ends (49 KiB).  Every grid fits, because
+// runs on, held until the process ends (192 KiB).  Every grid fits, because
 // the first pass never launches more than kNormsBlocks workgroups.  All calls
 // on one GPU share it: calls that overlap on one GPU race.
 static __device__ __align__(256) NormAcc scratch[kNormsBlocks + 1];
@@ -252,19 +253,26 @@ hipError_t step_norms(const Grid& g, const float* rho, const float* u, const flo
 {
     return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step, st);
 }
-hipError_t step_norms(const Grid& g, const bf16s* rho, const bf16s* u, const bf16s* v,
-                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
-                      const float* d, int rhs, const uint8_t* flags, void* scratch,
-                      smoke_step_norms* out, float dt, long long step, hipStream_t st)
-{
-    return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step, st);
-}
-hipError_t step_norms(const Grid& g, const float* rho, const bf16s* u, const bf16s* v,
-                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
-                      const float* d, int rhs, const uint8_t* flags, void* scratch,
-                      smoke_step_norms* out, float dt, long long step, hipStream_t st)
-{
-    return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step, st);
-}
+
+#define SMK_HALF_NORMS_DEFS(H)                                                                     \
+    hipError_t step_norms(const Grid& g, const H* rho, const H* u, const H* v, const H* w,         \
+                          const H* un, const H* vn, const H* wn, const float* d, int rhs,          \
+                          const uint8_t* flags, void* scratch, smoke_step_norms* out, float dt,    \
+                          long long step, hipStream_t st)                                          \
+    {                                                                                              \
+        return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step,    \
+                            st);                                                                   \
+    }                                                                                              \
+    hipError_t step_norms(const Grid& g, const float* rho, const H* u, const H* v, const H* w,     \
+                          const H* un, const H* vn, const H* wn, const float* d, int rhs,          \
+                          const uint8_t* flags, void* scratch, smoke_step_norms* out, float dt,    \
+                          long long step, hipStream_t st)                                          \
+    {                                                                                              \
+        return launch_norms(g, rho, u, v, w, un, vn, wn, d, rhs, flags, scratch, out, dt, step,    \
+                            st);                                                                   \
+    }
+SMK_HALF_NORMS_DEFS(bf16s)
+SMK_HALF_NORMS_DEFS(f16s)
+#undef SMK_HALF_NORMS_DEFS
 
 }  // namespace smk

@@ -3,13 +3,20 @@
 //
 // smoke_proxy_geometry_device enqueues the kernel on the caller's stream into
 // the caller's device arrays.  smoke_proxy_geometry serves host arrays: it
-// keeps, per device, one non-blocking stream and one pinned host workspace
-// holding the largest view asked for so far; the kernel writes its arrays
-// straight into that pinned memory (a few KB per view), so a view costs one
-// launch and one wait, then a host copy of exactly the vertices and indices
-// produced into the caller's arrays.
+// keeps, per device, one non-blocking stream and one pinned, host-coherent
+// workspace holding the largest view asked for so far; the kernel writes its
+// arrays straight into that memory (a few KB per view) and, last, a sequence
+// number the host waits for by polling the workspace (a stream
+// synchronisation wakes the caller tens of microseconds after the kernel
+// ends; the poll sees the number about a microsecond after it is written).
+// The poll gives up after kPollLimit and falls back to synchronising the
+// stream, which also reports a failed kernel.  Then the vertices and indices
+// produced are copied into the caller's arrays.
 #include <hip/hip_runtime.h>
 
+#include <immintrin.h>
+
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -25,7 +32,10 @@ struct ProxyWorkspace {
     hipStream_t stream = nullptr;
     char* mem = nullptr;  // pinned host memory the kernel writes
     int steps = 0;        // capacity in planes
+    unsigned seq = 0;     // number of the last view asked for
 };
+
+constexpr std::chrono::milliseconds kPollLimit{20};
 
 std::mutex g_proxy_mu;
 std::vector<ProxyWorkspace> g_proxy_ws;
@@ -65,12 +75,16 @@ smoke_status workspace(int steps, ProxyWorkspace** out)
         HIP_TRY(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
     }
     if (ws->steps < steps) {
-        if (ws->mem) HIP_TRY(hipHostFree(ws->mem));
+        if (ws->mem) {
+            HIP_TRY(hipStreamSynchronize(ws->stream));  // no kernel still writes it
+            HIP_TRY(hipHostFree(ws->mem));
+        }
         ws->mem = nullptr;
         ws->steps = 0;
         const size_t bytes = align256(vert_bytes(steps)) + align256(index_bytes(steps)) +
                              align256((size_t)steps * sizeof(smoke_proxy_plane)) + 256;
-        HIP_TRY(hipHostMalloc((void**)&ws->mem, bytes, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&ws->mem, bytes, hipHostMallocCoherent));
+        std::memset(ws->mem, 0, bytes);  // no stale completion mark
         ws->steps = steps;
     }
     *out = ws;
@@ -115,8 +129,19 @@ smoke_status smoke_proxy_geometry(const float box_position[3], const float box_d
     smoke_proxy_plane* dp = reinterpret_cast<smoke_proxy_plane*>(p);
     p += align256((size_t)ws->steps * sizeof(smoke_proxy_plane));
     int* dc = reinterpret_cast<int*>(p);
-    HIP_TRY(smk::proxy_geometry(v, steps, dv, di, dp, dc, ws->stream));
-    HIP_TRY(hipStreamSynchronize(ws->stream));
+    unsigned* done = reinterpret_cast<unsigned*>(p + 64);
+    const unsigned seq = ++ws->seq == 0 ? ++ws->seq : ws->seq;  // 0 never marks a view
+    HIP_TRY(smk::proxy_geometry(v, steps, dv, di, dp, dc, ws->stream, done, seq));
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(done, __ATOMIC_ACQUIRE) != seq) {
+        if (std::chrono::steady_clock::now() - t0 > kPollLimit) {
+            HIP_TRY(hipStreamSynchronize(ws->stream));
+            if (__atomic_load_n(done, __ATOMIC_ACQUIRE) != seq)
+                return fail(SMOKE_ERR_HIP, "proxy_geometry: the kernel ended without its completion mark");
+            break;
+        }
+        _mm_pause();
+    }
     const int counts[2] = {dc[0], dc[1]};
     std::memcpy(vertices, dv, (size_t)counts[0] * 3 * sizeof(float));
     std::memcpy(indices, di, (size_t)counts[1] * sizeof(int16_t));

@@ -4,8 +4,8 @@
 // (:356-361), sortPoints (:259-272), pseudoangle (:87-92) and teselate
 // (:243-255), computed on the device instead of sequentially on the CPU.
 //
-// One workgroup of kProxyThreads lanes; lane j of chunk c builds plane
-// k = c * kProxyThreads + j: its edge points (at most 12, plus their mean) in
+// One workgroup of nt = min(1024, steps rounded up to 64) lanes; lane j of
+// chunk c builds plane k = c * nt + j: its edge points (at most 12, plus their mean) in
 // registers, sorted with the same insertion sort, then a workgroup prefix sum
 // over the per-plane vertex and index counts places every plane's vertices,
 // fan triangles and (num_triangles, triangle_offset) entry exactly where the
@@ -14,6 +14,8 @@
 // / sqrt(dot(v, v))), cross, component-wise vec3 / scalar); the file is
 // compiled with -ffp-contract=off and HIP's correctly rounded division and
 // square root, so the vertices are bit-identical to the CPU oracle's.
+#include <algorithm>
+
 #include "smoke_internal.h"
 
 namespace smk {
@@ -163,15 +165,19 @@ __device__ void insertion_sort(Poly& P, float (&key)[kMaxPoints])
     }
 }
 
-constexpr int kProxyThreads = 256;
+constexpr int kProxyThreads = 1024;
 
+// `done` (may be NULL): after every output is written and made visible to
+// the host (system-scope fences), lane 0 stores `seq` there with a
+// system-scope release, so a host waiting on pinned memory may read the
+// outputs as soon as it sees the value.
 __global__ __launch_bounds__(kProxyThreads) void k_proxy_geometry(
     ProxyView view, int steps, float* __restrict__ verts, int16_t* __restrict__ indices,
-    smoke_proxy_plane* __restrict__ planes, int* __restrict__ counts)
+    smoke_proxy_plane* __restrict__ planes, int* __restrict__ counts, unsigned* done, unsigned seq)
 {
     __shared__ int scan_v[kProxyThreads];
     __shared__ int scan_i[kProxyThreads];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const V3 bp{view.box_pos[0], view.box_pos[1], view.box_pos[2]};
     const V3 dims{view.box_dims[0], view.box_dims[1], view.box_dims[2]};
     const V3 origin{view.origin[0], view.origin[1], view.origin[2]};
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(kProxyThreads) void k_proxy_geometry(
     const V3 end = add(origin, muls(dir, hi));
 
     int vbase = 0, ibase = 0;
-    for (int c0 = 0; c0 < steps; c0 += kProxyThreads) {
+    for (int c0 = 0; c0 < steps; c0 += nt) {
         const int k = c0 + tid;
         V3 q[kEdges];
         bool keep[kEdges];
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(kProxyThreads) void k_proxy_geometry(
         scan_v[tid] = nv;
         scan_i[tid] = ni;
         __syncthreads();
-        for (int off = 1; off < kProxyThreads; off <<= 1) {
+        for (int off = 1; off < nt; off <<= 1) {
             const int av = tid >= off ? scan_v[tid - off] : 0;
             const int ai = tid >= off ? scan_i[tid - off] : 0;
             __syncthreads();
@@ -274,23 +280,31 @@ __global__ __launch_bounds__(kProxyThreads) void k_proxy_geometry(
             planes[k].num_triangles = npts;
             planes[k].triangle_offset = io / 3;
         }
-        vbase += scan_v[kProxyThreads - 1];
-        ibase += scan_i[kProxyThreads - 1];
+        vbase += scan_v[nt - 1];
+        ibase += scan_i[nt - 1];
         __syncthreads();  // every lane has read the totals before they are overwritten
     }
     if (tid == 0) {
         counts[0] = vbase;
         counts[1] = ibase;
     }
+    if (done) {
+        __threadfence_system();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 }  // namespace
 
 hipError_t proxy_geometry(const ProxyView& view, int steps, float* verts, int16_t* indices,
-                          smoke_proxy_plane* planes, int* counts, hipStream_t st)
+                          smoke_proxy_plane* planes, int* counts, hipStream_t st, unsigned* done,
+                          unsigned seq)
 {
-    hipLaunchKernelGGL(k_proxy_geometry, dim3(1), dim3(kProxyThreads), 0, st, view, steps, verts,
-                       indices, planes, counts);
+    if (steps < 1) return hipErrorInvalidValue;
+    const int nt = std::min(kProxyThreads, (steps + 63) / 64 * 64);
+    hipLaunchKernelGGL(k_proxy_geometry, dim3(1), dim3(nt), 0, st, view, steps, verts, indices,
+                       planes, counts, done, seq);
     return hipGetLastError();
 }
 

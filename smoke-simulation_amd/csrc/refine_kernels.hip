@@ -322,7 +322,7 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
         const int n = T * 2 * (TY + 2) * (TXP + 2);
         for (int k = (int)(threadIdx.x + TXP * threadIdx.y); k < n; k += TXP * TY) flat[k] = 0u;
         for (int f = (int)(threadIdx.x + TXP * threadIdx.y); f < 256; f += TXP * TY)
-            sh.itab[f] = __builtin_bit_cast(uint16_t, (_Float16)dev::cell_inv((uint8_t)f));
+            sh.itab[f] = __builtin_bit_cast(uint16_t, dev::to_f16(dev::cell_inv((uint8_t)f)));
     }
     __syncthreads();
     // XCD-contiguous tile order, as in the FP32 fused kernel.
@@ -397,8 +397,8 @@ __global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t*
         h2 qv = ok ? nq[0] : zero2;
         if (FIRST) {
             const float2 rv = ok ? nr[0] : make_float2(0.0f, 0.0f);
-            qv.x = (_Float16)(rv.x * s * (float)ii.x);
-            qv.y = (_Float16)(rv.y * s * (float)ii.y);
+            qv.x = dev::to_f16(rv.x * s * (float)ii.x);
+            qv.y = dev::to_f16(rv.y * s * (float)ii.y);
         }
 #pragma unroll
         for (int j = 0; j + 1 < D; ++j) {

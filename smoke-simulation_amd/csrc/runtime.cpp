@@ -58,17 +58,19 @@ Grid make_grid(unsigned nx, unsigned ny, unsigned nz)
     return g;
 }
 
-constexpr int kStoredFields = 8;  // rho0 rho1 u0 u1 v0 v1 w0 w1 (FP32 or BF16)
+constexpr int kStoredFields = 8;  // rho0 rho1 u0 u1 v0 v1 w0 w1 (FP32 or 16-bit)
 constexpr int kFloatFields = 3;   // P0 P1 d (always FP32)
 constexpr int kByteFields = 3;    // base s flags
 
 bool storage_ok(int storage)
 {
-    return storage == SMOKE_STORAGE_FP32 || storage == SMOKE_STORAGE_BF16 ||
-           storage == SMOKE_STORAGE_BF16_VELOCITY;
+    return storage >= SMOKE_STORAGE_FP32 && storage <= SMOKE_STORAGE_FP16_VELOCITY;
 }
 // Bytes per element of the stored density and velocity fields.
-size_t density_bytes(int storage) { return storage == SMOKE_STORAGE_BF16 ? 2 : 4; }
+size_t density_bytes(int storage)
+{
+    return storage == SMOKE_STORAGE_BF16 || storage == SMOKE_STORAGE_FP16 ? 2 : 4;
+}
 size_t velocity_bytes(int storage) { return storage == SMOKE_STORAGE_FP32 ? 4 : 2; }
 
 // Arena: stored fields, P0 P1 d, staging (reference-layout faces, FP32), then
@@ -694,7 +696,7 @@ smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float*
     s->g = make_grid(nx, ny, nz);
     s->storage = storage;
     smoke_params_default(&s->prm);
-    // BF16 velocity cannot run the face-based reference projection.
+    // 16-bit velocity cannot run the face-based reference projection.
     if (storage != SMOKE_STORAGE_FP32) s->prm.projection = SMOKE_PROJ_JACOBI;
     auto bail = [&](smoke_status st) {
         smoke_destroy(s);
@@ -716,7 +718,11 @@ smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float*
     for (int k = 0; k < kStoredFields; ++k) {
         const int h = k & 1, field = k >> 1;  // rho, u, v, w
         const bool half = field == 0 ? density_bytes(storage) == 2 : velocity_bytes(storage) == 2;
-        if (half) {
+        const bool fp16 = storage == SMOKE_STORAGE_FP16 || storage == SMOKE_STORAGE_FP16_VELOCITY;
+        if (half && fp16) {
+            smk::f16s** f[4] = {s->frho, s->fu, s->fv, s->fw};
+            f[field][h] = reinterpret_cast<smk::f16s*>(p);
+        } else if (half) {
             smk::bf16s** f[4] = {s->hrho, s->hu, s->hv, s->hw};
             f[field][h] = reinterpret_cast<smk::bf16s*>(p);
         } else {
@@ -738,7 +744,7 @@ smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float*
         p += align256((size_t)s->g.elems);
     }
     // Zero everything (the reference leaves half [1] uninitialised; zero is
-    // the defined value here; all-zero bits are +0 in FP32 and BF16), then
+    // the defined value here; all-zero bits are +0 in FP32, BF16 and FP16), then
     // the floor mask and the initial density.
     e = smk::fill_u8(reinterpret_cast<uint8_t*>(s->arena), (long long)bytes, 0, s->stream);
     if (e == hipSuccess) e = smk::floor_mask(s->g, s->base, s->stream);
@@ -878,7 +884,7 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
     SMOKE_REFUSE_IN_BATCH(s, "set_params");
     if (s->storage != SMOKE_STORAGE_FP32 && p->projection == SMOKE_PROJ_REFERENCE)
         return fail(SMOKE_ERR_INVALID,
-                    "set_params: BF16 storage runs the pressure forms (RBSOR, JACOBI, REFINED), not REFERENCE");
+                    "set_params: 16-bit storage runs the pressure forms (RBSOR, JACOBI, REFINED), not REFERENCE");
     if (p->obstacle_mode != s->prm.obstacle_mode) s->mask_dirty = true;
     if (p->projection == SMOKE_PROJ_REFINED) {
         smoke_status rs = ensure_refine(s);

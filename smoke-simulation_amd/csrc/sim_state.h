@@ -44,14 +44,18 @@ struct smoke_sim {
     float* w[2] = {nullptr, nullptr};
     float* P[2] = {nullptr, nullptr};
     float* d = nullptr;
-    // SMOKE_STORAGE_BF16: density and u, v, w live in these instead (the
-    // float pointers above are then NULL); SMOKE_STORAGE_BF16_VELOCITY: u, v,
-    // w live here and the density in rho.
+    // SMOKE_STORAGE_BF16 (FP16): density and u, v, w live in hrho, hu, hv, hw
+    // (frho, fu, fv, fw) instead (the float pointers above are then NULL);
+    // the *_VELOCITY forms: u, v, w live there and the density in rho.
     int storage = SMOKE_STORAGE_FP32;
     smk::bf16s* hrho[2] = {nullptr, nullptr};
     smk::bf16s* hu[2] = {nullptr, nullptr};
     smk::bf16s* hv[2] = {nullptr, nullptr};
     smk::bf16s* hw[2] = {nullptr, nullptr};
+    smk::f16s* frho[2] = {nullptr, nullptr};
+    smk::f16s* fu[2] = {nullptr, nullptr};
+    smk::f16s* fv[2] = {nullptr, nullptr};
+    smk::f16s* fw[2] = {nullptr, nullptr};
     // REFINED solver buffers (allocated when the projection is first set to
     // REFINED).
     char* refine_mem = nullptr;
@@ -123,13 +127,18 @@ struct smoke_sim {
 
 namespace smk {
 // Calls fn(rho, u, v, w) with the simulation's field pointer pairs typed by
-// its storage (float* [2] or bf16s* [2], density and velocity each).
+// its storage (float* [2], bf16s* [2] or f16s* [2], density and velocity
+// each).
 template <typename Fn>
 auto with_fields(smoke_sim* s, Fn&& fn)
 {
-    if (s->storage == SMOKE_STORAGE_BF16) return fn(s->hrho, s->hu, s->hv, s->hw);
-    if (s->storage == SMOKE_STORAGE_BF16_VELOCITY) return fn(s->rho, s->hu, s->hv, s->hw);
-    return fn(s->rho, s->u, s->v, s->w);
+    switch (s->storage) {
+    case SMOKE_STORAGE_BF16: return fn(s->hrho, s->hu, s->hv, s->hw);
+    case SMOKE_STORAGE_BF16_VELOCITY: return fn(s->rho, s->hu, s->hv, s->hw);
+    case SMOKE_STORAGE_FP16: return fn(s->frho, s->fu, s->fv, s->fw);
+    case SMOKE_STORAGE_FP16_VELOCITY: return fn(s->rho, s->fu, s->fv, s->fw);
+    default: return fn(s->rho, s->u, s->v, s->w);
+    }
 }
 // Enqueues the step-norms reduction of the simulation's current state on its
 // stream, with `scratch` (norms_scratch_bytes()) for the partial records and

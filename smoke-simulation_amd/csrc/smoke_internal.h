@@ -32,9 +32,12 @@ struct Grid {
 constexpr int kBlockX = 64;
 constexpr int kBlockY = 4;
 
-// BF16 storage element of the mixed-precision form (the upper half of an
-// FP32 word; arithmetic stays FP32).
+// 16-bit storage elements of the mixed-precision forms (arithmetic stays
+// FP32): BF16 (the upper half of an FP32 word) and IEEE binary16.
 struct bf16s {
+    uint16_t bits;
+};
+struct f16s {
     uint16_t bits;
 };
 
@@ -216,16 +219,16 @@ hipError_t build_owner(const Grid& g, uint8_t* owner, const SphereList& obst, in
                        hipStream_t st);
 hipError_t obstacle_velocity_bc(const Grid& g, float* u, float* v, float* w, const uint8_t* owner,
                                 const SphereList& vel, hipStream_t st);
-hipError_t obstacle_velocity_bc(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* owner,
-                                const SphereList& vel, hipStream_t st);
 
 // Proxy-plane slicing geometry (proxy_kernels.hip): one workgroup builds
-// every plane of one view.
+// every plane of one view.  With `done` non-NULL the kernel stores `seq`
+// there (system-scope release) once every output is visible to the host.
 struct ProxyView {
     float box_pos[3], box_dims[3], origin[3], dir[3];
 };
 hipError_t proxy_geometry(const ProxyView& view, int steps, float* verts, int16_t* indices,
-                          smoke_proxy_plane* planes, int* counts, hipStream_t st);
+                          smoke_proxy_plane* planes, int* counts, hipStream_t st,
+                          unsigned* done = nullptr, unsigned seq = 0);
 
 // Vorticity confinement on [now], after the force stage (an extension; three
 // passes over four FP32 padded scratch fields).
@@ -234,8 +237,6 @@ struct VortBufs {
     float* f[3];  // omega, then the confinement force, per cell
 };
 hipError_t vorticity(const Grid& g, float* u, float* v, float* w, const uint8_t* flags,
-                     const VortBufs& b, float eps, float dt, hipStream_t st);
-hipError_t vorticity(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* flags,
                      const VortBufs& b, float eps, float dt, hipStream_t st);
 
 // Projection.
@@ -263,37 +264,39 @@ hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float
                           const float* v, const float* w, const uint8_t* flags, float dt,
                           hipStream_t st);
 
-// BF16-stored fields (FP32 arithmetic): the same stages and conversions.
-hipError_t unpack_cells_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st);
-hipError_t pack_cells_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st);
-hipError_t unpack_faces_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st);
-hipError_t pack_faces_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st);
-hipError_t pack_cells_half(const Grid& g, const bf16s* src, uint16_t* dst, int format,
-                           hipStream_t st);
-hipError_t fill_source(const Grid& g, bf16s* rho0, bf16s* rho1, float4 sph, hipStream_t st,
-                       int* launched);
-hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const bf16s* rho,
-                 const uint8_t* flags, const ForceArgs& a, hipStream_t st);
-hipError_t divergence(const Grid& g, const bf16s* u, const bf16s* v, const bf16s* w,
-                      const uint8_t* flags, float* d, hipStream_t st);
-hipError_t gradient(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* p,
-                    const uint8_t* flags, hipStream_t st);
-hipError_t advect_velocity(const Grid& g, const bf16s* u0, const bf16s* v0, const bf16s* w0,
-                           bf16s* u1, bf16s* v1, bf16s* w1, const uint8_t* flags, float dt,
-                           hipStream_t st);
-hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s* u,
-                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
-                          hipStream_t st);
-// FP32 density with BF16 velocity (the split storage of the mixed form).
-hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* rho,
-                 const uint8_t* flags, const ForceArgs& a, hipStream_t st);
-hipError_t advect_density(const Grid& g, const float* r0, float* r1, const bf16s* u,
-                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
-                          hipStream_t st);
-hipError_t step_norms(const Grid& g, const float* rho, const bf16s* u, const bf16s* v,
-                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
-                      const float* d, int rhs, const uint8_t* flags, void* scratch,
-                      smoke_step_norms* out, float dt, long long step, hipStream_t st);
+// 16-bit stored fields H (bf16s or f16s; FP32 arithmetic): the same stages
+// and conversions, with density stored as H or as FP32 (the split forms).
+// The step-norms reduction is declared with the FP32 one below.
+#define SMK_HALF_STORAGE_DECLS(H)                                                                  \
+    hipError_t unpack_cells_f32(const Grid& g, const float* src, H* dst, hipStream_t st);          \
+    hipError_t pack_cells_f32(const Grid& g, const H* src, float* dst, hipStream_t st);            \
+    hipError_t unpack_faces_f32(const Grid& g, const float* src, H* dst, hipStream_t st);          \
+    hipError_t pack_faces_f32(const Grid& g, const H* src, float* dst, hipStream_t st);            \
+    hipError_t pack_cells_half(const Grid& g, const H* src, uint16_t* dst, int format,             \
+                               hipStream_t st);                                                    \
+    hipError_t fill_source(const Grid& g, H* rho0, H* rho1, float4 sph, hipStream_t st,            \
+                           int* launched);                                                         \
+    hipError_t force(const Grid& g, H* u, H* v, H* w, const H* rho, const uint8_t* flags,          \
+                     const ForceArgs& a, hipStream_t st);                                          \
+    hipError_t force(const Grid& g, H* u, H* v, H* w, const float* rho, const uint8_t* flags,      \
+                     const ForceArgs& a, hipStream_t st);                                          \
+    hipError_t obstacle_velocity_bc(const Grid& g, H* u, H* v, H* w, const uint8_t* owner,         \
+                                    const SphereList& vel, hipStream_t st);                        \
+    hipError_t vorticity(const Grid& g, H* u, H* v, H* w, const uint8_t* flags,                    \
+                         const VortBufs& b, float eps, float dt, hipStream_t st);                  \
+    hipError_t divergence(const Grid& g, const H* u, const H* v, const H* w,                       \
+                          const uint8_t* flags, float* d, hipStream_t st);                         \
+    hipError_t gradient(const Grid& g, H* u, H* v, H* w, const float* p, const uint8_t* flags,     \
+                        hipStream_t st);                                                           \
+    hipError_t advect_velocity(const Grid& g, const H* u0, const H* v0, const H* w0, H* u1,        \
+                               H* v1, H* w1, const uint8_t* flags, float dt, hipStream_t st);      \
+    hipError_t advect_density(const Grid& g, const H* r0, H* r1, const H* u, const H* v,           \
+                              const H* w, const uint8_t* flags, float dt, hipStream_t st);         \
+    hipError_t advect_density(const Grid& g, const float* r0, float* r1, const H* u, const H* v,   \
+                              const H* w, const uint8_t* flags, float dt, hipStream_t st);
+SMK_HALF_STORAGE_DECLS(bf16s)
+SMK_HALF_STORAGE_DECLS(f16s)
+#undef SMK_HALF_STORAGE_DECLS
 
 // ------------------------------------------------------------ step norms
 // Result record of the norms reduction (norms_kernels.hip), read back by the
@@ -307,7 +310,7 @@ struct StepNorms {
 };
 // Workgroups of the first reduction pass (at most): the scratch holds that
 // many partial records plus the reduced one (norms_scratch_bytes()).
-constexpr int kNormsBlocks = 1024;
+constexpr int kNormsBlocks = 4096;
 size_t norms_scratch_bytes();
 // Enqueues the reduction (partials, fixed-order total, then the public
 // record written to `out`, device memory).  rhs = 0: d is not read (no
@@ -317,10 +320,18 @@ hipError_t step_norms(const Grid& g, const float* rho, const float* u, const flo
                       const float* w, const float* un, const float* vn, const float* wn,
                       const float* d, int rhs, const uint8_t* flags, void* scratch,
                       smoke_step_norms* out, float dt, long long step, hipStream_t st);
-hipError_t step_norms(const Grid& g, const bf16s* rho, const bf16s* u, const bf16s* v,
-                      const bf16s* w, const bf16s* un, const bf16s* vn, const bf16s* wn,
-                      const float* d, int rhs, const uint8_t* flags, void* scratch,
-                      smoke_step_norms* out, float dt, long long step, hipStream_t st);
+#define SMK_HALF_NORMS_DECLS(H)                                                                    \
+    hipError_t step_norms(const Grid& g, const H* rho, const H* u, const H* v, const H* w,         \
+                          const H* un, const H* vn, const H* wn, const float* d, int rhs,          \
+                          const uint8_t* flags, void* scratch, smoke_step_norms* out, float dt,    \
+                          long long step, hipStream_t st);                                         \
+    hipError_t step_norms(const Grid& g, const float* rho, const H* u, const H* v, const H* w,     \
+                          const H* un, const H* vn, const H* wn, const float* d, int rhs,          \
+                          const uint8_t* flags, void* scratch, smoke_step_norms* out, float dt,    \
+                          long long step, hipStream_t st);
+SMK_HALF_NORMS_DECLS(bf16s)
+SMK_HALF_NORMS_DECLS(f16s)
+#undef SMK_HALF_NORMS_DECLS
 // Device address (on `device`) of the library's static norms scratch, for
 // the allocation-free entry point; resolved once per device.
 hipError_t noalloc_norms_scratch(int device, void** out);

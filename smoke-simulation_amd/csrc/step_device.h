@@ -41,9 +41,10 @@ __device__ __forceinline__ bool is_interior(const Grid& g, int x, int y, int z)
     return x >= 1 && x <= g.nx - 2 && y >= 1 && y <= g.ny - 2 && z >= 1 && z <= g.nz - 2;
 }
 
-// Field storage: FP32, or BF16 (mixed-precision form) widened to FP32 on load
-// and rounded to nearest even on store.  The step bodies below are templates
-// over the element type; with float they are plain loads and stores.
+// Field storage: FP32, or BF16 / FP16 (mixed-precision forms) widened to FP32
+// on load (exactly) and rounded to nearest even on store.  The step bodies
+// below are templates over the element type; with float they are plain loads
+// and stores.
 __device__ __forceinline__ float ldf(const float* __restrict__ p, int i) { return p[i]; }
 __device__ __forceinline__ float ldf(const bf16s* __restrict__ p, int i)
 {
@@ -59,6 +60,30 @@ __device__ __forceinline__ void stf(float* __restrict__ p, int i, float v) { p[i
 __device__ __forceinline__ void stf(bf16s* __restrict__ p, int i, float v)
 {
     p[i].bits = bf16_rne(v);
+}
+// An FP32 value the compiler must treat as opaque: the FP16 conversion that
+// follows stays a conversion of this FP32 value.  Without it the compiler may
+// fuse the multiply or add that produced the value into the conversion
+// (v_fma_mix*_f16), which rounds the exact result once to FP16 instead of
+// the FP32 result, so the same source would give different bits in
+// different kernels.
+#ifndef SMK_OPAQUE_F32
+#define SMK_OPAQUE_F32(x) asm volatile("" : "+v"(x))
+#endif
+__device__ __forceinline__ _Float16 to_f16(float v)
+{
+    SMK_OPAQUE_F32(v);
+    return (_Float16)v;
+}
+// binary16: the hardware conversions (FP32 -> FP16 rounds to nearest even,
+// keeps subnormals, overflows to Inf and keeps NaN).
+__device__ __forceinline__ float ldf(const f16s* __restrict__ p, int i)
+{
+    return (float)__builtin_bit_cast(_Float16, p[i].bits);
+}
+__device__ __forceinline__ void stf(f16s* __restrict__ p, int i, float v)
+{
+    p[i].bits = __builtin_bit_cast(uint16_t, to_f16(v));
 }
 
 // fillSmoke (smokeSimulation.cu:251-273) for one cell the caller knows to be
@@ -619,6 +644,13 @@ __device__ __forceinline__ float2 ld2(const bf16s* __restrict__ a, int r)
     uint32_t w;  // both values with one 4-byte load (2-byte aligned)
     __builtin_memcpy(&w, a + r, sizeof w);
     return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
+__device__ __forceinline__ float2 ld2(const f16s* __restrict__ a, int r)
+{
+    uint32_t w;
+    __builtin_memcpy(&w, a + r, sizeof w);
+    return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)),
+                       (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)));
 }
 
 // The eight corners in the reference's order (:473-481), from the four

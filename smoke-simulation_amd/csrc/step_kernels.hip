@@ -103,8 +103,7 @@ __global__ void k_pack_half(Grid g, const T* __restrict__ src, uint16_t* __restr
     const float f = dev::ldf(src, g.idx(x, y, z));
     uint16_t h;
     if (format == 1) {
-        const _Float16 v = (_Float16)f;
-        h = __builtin_bit_cast(uint16_t, v);
+        h = __builtin_bit_cast(uint16_t, dev::to_f16(f));
     } else {
         h = dev::bf16_rne(f);
     }
@@ -492,38 +491,7 @@ hipError_t pack_cells_half(const Grid& g, const float* src, uint16_t* dst, int f
 {
     return launch_pack_half(g, src, dst, format, st);
 }
-hipError_t pack_cells_half(const Grid& g, const bf16s* src, uint16_t* dst, int format,
-                           hipStream_t st)
-{
-    return launch_pack_half(g, src, dst, format, st);
-}
 
-hipError_t unpack_cells_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_convert<true, bf16s>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, g,
-                       (const void*)src, (void*)dst, g.nx, g.ny, g.nz);
-    return hipGetLastError();
-}
-hipError_t pack_cells_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_convert<false, bf16s>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,
-                       g, (const void*)src, (void*)dst, g.nx, g.ny, g.nz);
-    return hipGetLastError();
-}
-hipError_t unpack_faces_f32(const Grid& g, const float* src, bf16s* dst, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_convert<true, bf16s>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),
-                       dim3(BX, BY), 0, st, g, (const void*)src, (void*)dst, g.nx + 1, g.ny + 1,
-                       g.nz + 1);
-    return hipGetLastError();
-}
-hipError_t pack_faces_f32(const Grid& g, const bf16s* src, float* dst, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_convert<false, bf16s>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),
-                       dim3(BX, BY), 0, st, g, (const void*)src, (void*)dst, g.nx + 1, g.ny + 1,
-                       g.nz + 1);
-    return hipGetLastError();
-}
 
 hipError_t floor_mask(const Grid& g, uint8_t* base, hipStream_t st)
 {
@@ -556,11 +524,6 @@ static hipError_t launch_obstacle_bc(const Grid& g, T* u, T* v, T* w, const uint
     return hipGetLastError();
 }
 hipError_t obstacle_velocity_bc(const Grid& g, float* u, float* v, float* w, const uint8_t* owner,
-                                const SphereList& vel, hipStream_t st)
-{
-    return launch_obstacle_bc(g, u, v, w, owner, vel, st);
-}
-hipError_t obstacle_velocity_bc(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* owner,
                                 const SphereList& vel, hipStream_t st)
 {
     return launch_obstacle_bc(g, u, v, w, owner, vel, st);
@@ -609,11 +572,6 @@ hipError_t fill_source(const Grid& g, float* rho0, float* rho1, float4 sph, hipS
 {
     return launch_source(g, rho0, rho1, sph, st, launched);
 }
-hipError_t fill_source(const Grid& g, bf16s* rho0, bf16s* rho1, float4 sph, hipStream_t st,
-                       int* launched)
-{
-    return launch_source(g, rho0, rho1, sph, st, launched);
-}
 
 template <typename T, typename TR>
 static hipError_t launch_force(const Grid& g, T* u, T* v, T* w, const TR* rho, const uint8_t* flags,
@@ -624,16 +582,6 @@ static hipError_t launch_force(const Grid& g, T* u, T* v, T* w, const TR* rho, c
     return hipGetLastError();
 }
 hipError_t force(const Grid& g, float* u, float* v, float* w, const float* rho,
-                 const uint8_t* flags, const ForceArgs& a, hipStream_t st)
-{
-    return launch_force(g, u, v, w, rho, flags, a, st);
-}
-hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const bf16s* rho,
-                 const uint8_t* flags, const ForceArgs& a, hipStream_t st)
-{
-    return launch_force(g, u, v, w, rho, flags, a, st);
-}
-hipError_t force(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* rho,
                  const uint8_t* flags, const ForceArgs& a, hipStream_t st)
 {
     return launch_force(g, u, v, w, rho, flags, a, st);
@@ -660,11 +608,6 @@ hipError_t vorticity(const Grid& g, float* u, float* v, float* w, const uint8_t*
 {
     return launch_vorticity(g, u, v, w, flags, b, eps, dt, st);
 }
-hipError_t vorticity(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const uint8_t* flags,
-                     const VortBufs& b, float eps, float dt, hipStream_t st)
-{
-    return launch_vorticity(g, u, v, w, flags, b, eps, dt, st);
-}
 
 hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
                                   const uint8_t* flags, int colour, double omega, hipStream_t st)
@@ -684,11 +627,6 @@ static hipError_t launch_divergence(const Grid& g, const T* u, const T* v, const
     return hipGetLastError();
 }
 hipError_t divergence(const Grid& g, const float* u, const float* v, const float* w,
-                      const uint8_t* flags, float* d, hipStream_t st)
-{
-    return launch_divergence(g, u, v, w, flags, d, st);
-}
-hipError_t divergence(const Grid& g, const bf16s* u, const bf16s* v, const bf16s* w,
                       const uint8_t* flags, float* d, hipStream_t st)
 {
     return launch_divergence(g, u, v, w, flags, d, st);
@@ -785,11 +723,6 @@ hipError_t gradient(const Grid& g, float* u, float* v, float* w, const float* p,
 {
     return launch_gradient(g, u, v, w, p, flags, st);
 }
-hipError_t gradient(const Grid& g, bf16s* u, bf16s* v, bf16s* w, const float* p,
-                    const uint8_t* flags, hipStream_t st)
-{
-    return launch_gradient(g, u, v, w, p, flags, st);
-}
 
 template <typename T>
 static hipError_t launch_advect_velocity(const Grid& g, const T* u0, const T* v0, const T* w0,
@@ -804,12 +737,6 @@ static hipError_t launch_advect_velocity(const Grid& g, const T* u0, const T* v0
 }
 hipError_t advect_velocity(const Grid& g, const float* u0, const float* v0, const float* w0,
                            float* u1, float* v1, float* w1, const uint8_t* flags, float dt,
-                           hipStream_t st)
-{
-    return launch_advect_velocity(g, u0, v0, w0, u1, v1, w1, flags, dt, st);
-}
-hipError_t advect_velocity(const Grid& g, const bf16s* u0, const bf16s* v0, const bf16s* w0,
-                           bf16s* u1, bf16s* v1, bf16s* w1, const uint8_t* flags, float dt,
                            hipStream_t st)
 {
     return launch_advect_velocity(g, u0, v0, w0, u1, v1, w1, flags, dt, st);
@@ -831,17 +758,93 @@ hipError_t advect_density(const Grid& g, const float* r0, float* r1, const float
 {
     return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
 }
-hipError_t advect_density(const Grid& g, const bf16s* r0, bf16s* r1, const bf16s* u,
-                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
-                          hipStream_t st)
-{
-    return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
-}
-hipError_t advect_density(const Grid& g, const float* r0, float* r1, const bf16s* u,
-                          const bf16s* v, const bf16s* w, const uint8_t* flags, float dt,
-                          hipStream_t st)
-{
-    return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);
-}
+
+// The stages and conversions for 16-bit stored fields H (declared in
+// smoke_internal.h), density stored as H or as FP32.
+#define SMK_HALF_STORAGE_DEFS(H)                                                                   \
+    hipError_t pack_cells_half(const Grid& g, const H* src, uint16_t* dst, int format,             \
+                               hipStream_t st)                                                     \
+    {                                                                                              \
+        return launch_pack_half(g, src, dst, format, st);                                          \
+    }                                                                                              \
+    hipError_t unpack_cells_f32(const Grid& g, const float* src, H* dst, hipStream_t st)           \
+    {                                                                                              \
+        hipLaunchKernelGGL((k_convert<true, H>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st,  \
+                           g, (const void*)src, (void*)dst, g.nx, g.ny, g.nz);                     \
+        return hipGetLastError();                                                                  \
+    }                                                                                              \
+    hipError_t pack_cells_f32(const Grid& g, const H* src, float* dst, hipStream_t st)             \
+    {                                                                                              \
+        hipLaunchKernelGGL((k_convert<false, H>), grid_for(g.nx, g.ny, g.nz), dim3(BX, BY), 0, st, \
+                           g, (const void*)src, (void*)dst, g.nx, g.ny, g.nz);                     \
+        return hipGetLastError();                                                                  \
+    }                                                                                              \
+    hipError_t unpack_faces_f32(const Grid& g, const float* src, H* dst, hipStream_t st)           \
+    {                                                                                              \
+        hipLaunchKernelGGL((k_convert<true, H>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),           \
+                           dim3(BX, BY), 0, st, g, (const void*)src, (void*)dst, g.nx + 1,         \
+                           g.ny + 1, g.nz + 1);                                                    \
+        return hipGetLastError();                                                                  \
+    }                                                                                              \
+    hipError_t pack_faces_f32(const Grid& g, const H* src, float* dst, hipStream_t st)             \
+    {                                                                                              \
+        hipLaunchKernelGGL((k_convert<false, H>), grid_for(g.nx + 1, g.ny + 1, g.nz + 1),          \
+                           dim3(BX, BY), 0, st, g, (const void*)src, (void*)dst, g.nx + 1,         \
+                           g.ny + 1, g.nz + 1);                                                    \
+        return hipGetLastError();                                                                  \
+    }                                                                                              \
+    hipError_t fill_source(const Grid& g, H* rho0, H* rho1, float4 sph, hipStream_t st,            \
+                           int* launched)                                                          \
+    {                                                                                              \
+        return launch_source(g, rho0, rho1, sph, st, launched);                                    \
+    }                                                                                              \
+    hipError_t force(const Grid& g, H* u, H* v, H* w, const H* rho, const uint8_t* flags,          \
+                     const ForceArgs& a, hipStream_t st)                                           \
+    {                                                                                              \
+        return launch_force(g, u, v, w, rho, flags, a, st);                                        \
+    }                                                                                              \
+    hipError_t force(const Grid& g, H* u, H* v, H* w, const float* rho, const uint8_t* flags,      \
+                     const ForceArgs& a, hipStream_t st)                                           \
+    {                                                                                              \
+        return launch_force(g, u, v, w, rho, flags, a, st);                                        \
+    }                                                                                              \
+    hipError_t obstacle_velocity_bc(const Grid& g, H* u, H* v, H* w, const uint8_t* owner,         \
+                                    const SphereList& vel, hipStream_t st)                         \
+    {                                                                                              \
+        return launch_obstacle_bc(g, u, v, w, owner, vel, st);                                     \
+    }                                                                                              \
+    hipError_t vorticity(const Grid& g, H* u, H* v, H* w, const uint8_t* flags,                    \
+                         const VortBufs& b, float eps, float dt, hipStream_t st)                   \
+    {                                                                                              \
+        return launch_vorticity(g, u, v, w, flags, b, eps, dt, st);                                \
+    }                                                                                              \
+    hipError_t divergence(const Grid& g, const H* u, const H* v, const H* w,                       \
+                          const uint8_t* flags, float* d, hipStream_t st)                          \
+    {                                                                                              \
+        return launch_divergence(g, u, v, w, flags, d, st);                                        \
+    }                                                                                              \
+    hipError_t gradient(const Grid& g, H* u, H* v, H* w, const float* p, const uint8_t* flags,     \
+                        hipStream_t st)                                                            \
+    {                                                                                              \
+        return launch_gradient(g, u, v, w, p, flags, st);                                          \
+    }                                                                                              \
+    hipError_t advect_velocity(const Grid& g, const H* u0, const H* v0, const H* w0, H* u1,        \
+                               H* v1, H* w1, const uint8_t* flags, float dt, hipStream_t st)       \
+    {                                                                                              \
+        return launch_advect_velocity(g, u0, v0, w0, u1, v1, w1, flags, dt, st);                   \
+    }                                                                                              \
+    hipError_t advect_density(const Grid& g, const H* r0, H* r1, const H* u, const H* v,           \
+                              const H* w, const uint8_t* flags, float dt, hipStream_t st)          \
+    {                                                                                              \
+        return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);                           \
+    }                                                                                              \
+    hipError_t advect_density(const Grid& g, const float* r0, float* r1, const H* u, const H* v,   \
+                              const H* w, const uint8_t* flags, float dt, hipStream_t st)          \
+    {                                                                                              \
+        return launch_advect_density(g, r0, r1, u, v, w, flags, dt, st);                           \
+    }
+SMK_HALF_STORAGE_DEFS(bf16s)
+SMK_HALF_STORAGE_DEFS(f16s)
+#undef SMK_HALF_STORAGE_DEFS
 
 }  // namespace smk

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libsmokesim.so")
 SMOKE_OK = 0
 ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_NONFINITE = 1, 2, 3, 4, 5
 PROJ_REFERENCE, PROJ_RBSOR, PROJ_JACOBI, PROJ_REFINED = 0, 1, 2, 3
-STORAGE_FP32, STORAGE_BF16, STORAGE_BF16_VELOCITY = 0, 1, 2
+STORAGE_FP32, STORAGE_BF16, STORAGE_BF16_VELOCITY, STORAGE_FP16, STORAGE_FP16_VELOCITY = 0, 1, 2, 3, 4
 OBSTACLES_LAST_WINS, OBSTACLES_UNION = 0, 1
 STAGE_SCENE, STAGE_FORCE, STAGE_PROJECT, STAGE_ADVECT_VEL, STAGE_ADVECT_DENS = range(5)
 STAGE_COUNT = 5
@@ -237,7 +237,8 @@ def footprint_bytes(nx, ny, nz, storage=STORAGE_FP32, refined=False):
 
 class Simulation:
     """One simulation on the current device (smoke_create_ex ... smoke_destroy).
-    storage: STORAGE_FP32 (the drop-in) or STORAGE_BF16 (mixed-precision form)."""
+    storage: STORAGE_FP32 (the drop-in) or a 16-bit form (STORAGE_FP16,
+    STORAGE_BF16; *_VELOCITY keeps the density in FP32)."""
 
     def __init__(self, nx, ny, nz, density=None, params=None, storage=STORAGE_FP32):
         self.lib = load()

@@ -163,7 +163,8 @@ def case_precision():
     o = oracle_for(sc, 64)
     o.step(sc["dt"], 2)
     ref = o.state()
-    for storage in (smokesim.STORAGE_FP32, smokesim.STORAGE_BF16):
+    for storage in (smokesim.STORAGE_FP32, smokesim.STORAGE_BF16, smokesim.STORAGE_FP16,
+                    smokesim.STORAGE_FP16_VELOCITY):
         g = gpu_for(sc, storage=storage)
         g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=6, iterations=12)
         for _ in range(2):
@@ -186,6 +187,11 @@ def case_precision():
     rne = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16).astype(np.uint32).view(np.float32)
     same_bits(g.get_density(0), rne, "bf16 store rounding")
     g.close()
+    # FP16 round trip: binary16 round to nearest even (numpy's conversion)
+    g = smokesim.Simulation(9, 7, 5, storage=smokesim.STORAGE_FP16)
+    g.set_density(0, x)
+    same_bits(g.get_density(0), x.astype(np.float16).astype(np.float32), "fp16 store rounding")
+    g.close()
     # tolerance-driven refinement on a small grid
     sc = scenes.test_scene((8, 7, 6), seed=8, projection=2, iterations=10, box_fraction=0.0,
                            n_obstacles=0)
@@ -196,6 +202,42 @@ def case_precision():
     rel, nref = g.solve_info()
     assert rel <= 1e-5 and 1 <= nref < 1000, (rel, nref)
     g.close()
+
+
+def case_refine_groups():
+    # The refined solve's FP16 sweeps give the same bits however they are
+    # grouped into fused launches (plans T = 2, 4, 6 split 10 inner sweeps as
+    # 2+2+2+2+2, 4+4+2 or 6+4).
+    sc = scenes.test_scene((21, 13, 11), seed=9, projection=2, iterations=20)
+    states = []
+    for t in (2, 4, 6):
+        g = gpu_for(sc, storage=smokesim.STORAGE_BF16_VELOCITY)
+        g.set_jacobi_plan(sweeps_per_launch=t, tile=0, zc=5)
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=10, iterations=20)
+        for _ in range(2):
+            g.step(sc["dt"], readback=False)
+        states.append((t, g.state(), g.get_pressure()))
+        g.close()
+    for t, st, pr in states[1:]:
+        assert_bitwise_equal(st, states[0][1], f"refined, plan T={t}")
+        same_bits(pr, states[0][2], f"refined pressure, plan T={t}")
+
+
+def case_proxy():
+    # Proxy-plane geometry: one and two chunks of the 1024-lane workgroup,
+    # through the host entry point (pinned workspace, completion mark).
+    import oracle as orc
+    rng = np.random.default_rng(5)
+    for steps in (1, 100, 1025):
+        dims = rng.integers(16, 160, 3).astype(np.float32)
+        view = (np.zeros(3, np.float32), dims, rng.uniform(-300, 300, 3).astype(np.float32),
+                rng.normal(size=3).astype(np.float32))
+        gv, gi, gp = smokesim.proxy_geometry(*view, steps)
+        ov, oi, op = orc.proxy_geometry(*view, steps)
+        assert np.array_equal(gi, oi) and np.array_equal(gp, op), steps
+        nan = np.isnan(ov)
+        assert np.array_equal(np.isnan(gv), nan), steps
+        assert np.array_equal(gv[~nan].view(np.uint32), ov[~nan].view(np.uint32)), steps
 
 
 def case_vorticity():
@@ -247,7 +289,7 @@ def case_norms():
     g.close()
 
 
-CASES = {"single": case_single, "norms": case_norms, "plans": case_plans, "batch": case_batch,
+CASES = {"proxy": case_proxy, "refine_groups": case_refine_groups, "single": case_single, "norms": case_norms, "plans": case_plans, "batch": case_batch,
          "server": case_server, "readback": case_readback, "precision": case_precision,
          "vorticity": case_vorticity, "advection": case_advection}
 

@@ -32,6 +32,19 @@ extern thread_local EmuIdx threadIdx, blockIdx, blockDim, gridDim;
 void emu_syncthreads();
 #define __syncthreads() emu_syncthreads()
 
+// System-scope fence and release store: host threads share one coherent
+// memory, so a sequentially consistent fence and an atomic store suffice.
+inline void __threadfence_system() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+#ifndef __HIP_MEMORY_SCOPE_SYSTEM
+#define __HIP_MEMORY_SCOPE_SYSTEM 5
+#endif
+#define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), (order))
+
+// Host arithmetic rounds every FP32 operation before a conversion, so the
+// device's opaque-value barrier is not needed (and its "v" register
+// constraint does not exist on the host).
+#define SMK_OPAQUE_F32(x) ((void)(x))
+
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline unsigned __float_as_uint(float f)
 {

@@ -84,11 +84,11 @@ def test_c2_grid_512_two_fp32_steps_bitwise(scene512):
     g.close()
 
 
-# Mixed-precision form (FP32 density, BF16 velocity, FP16-inner refined
-# solve) against the FP32 form after 2 steps at 512^3 (same inputs, same
-# sweep count).
-C2_MIXED_DENSITY_REL_L2 = 1e-2
-C2_MIXED_VELOCITY_REL_L2 = 5e-2
+# Mixed-precision form (FP16 density and velocity, FP16-inner refined solve)
+# against the FP32 form after 2 steps at 512^3 (same inputs, same sweep
+# count).
+C2_MIXED_DENSITY_REL_L2 = 1e-3
+C2_MIXED_VELOCITY_REL_L2 = 1e-2
 
 
 def test_c2_grid_512_mixed_form_within_tolerance(scene512):
@@ -97,7 +97,7 @@ def test_c2_grid_512_mixed_form_within_tolerance(scene512):
     for _ in range(2):
         ref.step_async(sc["dt"])
     ref.synchronize()
-    m = gpu_for(sc, storage=smokesim.STORAGE_BF16_VELOCITY)
+    m = gpu_for(sc, storage=smokesim.STORAGE_FP16)
     m.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=20, iterations=100)
     for _ in range(2):
         m.step_async(sc["dt"])

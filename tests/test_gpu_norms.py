@@ -62,15 +62,17 @@ def test_norms_after_steps_equal_the_restatement(proj):
     g.close()
 
 
-def test_norms_of_bf16_fields():
+@pytest.mark.parametrize("storage", [smokesim.STORAGE_BF16, smokesim.STORAGE_FP16,
+                                     smokesim.STORAGE_FP16_VELOCITY])
+def test_norms_of_half_fields(storage):
     sc = scenes.test_scene((33, 21, 26), seed=22, projection=2, iterations=10, noise=0.3)
-    g = gpu_for(sc, storage=smokesim.STORAGE_BF16)
+    g = gpu_for(sc, storage=storage)
     for _ in range(2):
         g.step_async(sc["dt"])
     n = g.step_norms()
-    ref = orc.step_norms(g, sc["dt"])   # the BF16 values widen exactly
+    ref = orc.step_norms(g, sc["dt"])   # the 16-bit values widen exactly
     ref["rhs_l2"] = n["rhs_l2"]
-    check(n, ref, "bf16 fields")
+    check(n, ref, f"storage {storage}")
     assert n["rhs_l2"] > n["div_l2"] >= 0.0
     g.close()
 

@@ -1,6 +1,7 @@
-"""Mixed-precision form: BF16 storage of density and u, v, w (FP32
-arithmetic) and the REFINED projection (FP32 residual-correction loop around
-FP16 Jacobi sweeps, FP64 norms on the GPU), against the FP64 oracle.
+"""Mixed-precision forms: 16-bit storage (BF16 or FP16) of density and
+u, v, w (FP32 arithmetic) and the REFINED projection (FP32 residual-correction
+loop around FP16 Jacobi sweeps, FP64 norms on the GPU), against the FP64
+oracle.
 
 In exact arithmetic R refinements of n Jacobi sweeps from a zero correction
 are R*n Jacobi sweeps, so the FP64 oracle's Jacobi projection with R*n sweeps
@@ -9,6 +10,9 @@ is the reference; the differences below are rounding only.  Tolerances
   * FP32 fields + REFINED, 5 steps:  pressure and velocity <= 1e-3, density <= 1e-4
   * BF16 fields (JACOBI or REFINED), 5 steps: density <= 5e-3, velocity and
     pressure <= 2e-2 (BF16 keeps 8 significant bits: one rounding is <= 2^-9)
+  * FP16 fields (JACOBI or REFINED), 5 steps: density <= 1e-3, velocity and
+    pressure <= 3e-3 (binary16 keeps 11 significant bits: one rounding is
+    <= 2^-12, 8x finer than BF16)
   * the solid mask and the cell flags stay bit-exact in every form."""
 import numpy as np
 import pytest
@@ -67,6 +71,37 @@ def test_bf16_fields_within_tolerance(dims, seed, projection):
     g.close()
 
 
+@pytest.mark.parametrize("dims,seed", CASES[:2])
+@pytest.mark.parametrize("projection", [smokesim.PROJ_JACOBI, smokesim.PROJ_REFINED])
+@pytest.mark.parametrize("storage", [smokesim.STORAGE_FP16, smokesim.STORAGE_FP16_VELOCITY])
+def test_fp16_fields_within_tolerance(dims, seed, projection, storage):
+    g, o = run_pair(dims, seed, storage, projection)
+    err = field_errors(g, o)
+    print("fp16 errors", storage, dims, projection, err)
+    assert err["rho"] <= 1e-3, err
+    assert max(err["u"], err["v"], err["w"], err["pressure"]) <= 3e-3, err
+    g.close()
+
+
+def test_fp16_store_is_round_to_nearest_even():
+    # binary16 through set/get: round to nearest even (ties included),
+    # subnormals kept, -0 kept, the largest finite value kept and values past
+    # its rounding boundary overflowing to Inf.
+    g = smokesim.Simulation(19, 7, 5, storage=smokesim.STORAGE_FP16)
+    x = np.random.default_rng(5).standard_normal((5, 7, 19)).astype(np.float32)
+    x.flat[:9] = [1.0, -0.0, np.float32(1.0 + 2.0 ** -11), np.float32(1.0 + 3 * 2.0 ** -11),
+                  65504.0, 65519.0, 65520.0, np.float32(3e-6), np.float32(-7e-8)]
+    g.set_density(0, x)
+    got = g.get_density(0)
+    assert np.array_equal(got.view(np.uint32), _rne_f16_bits(x))
+    assert got.flat[0] == 1.0 and got.flat[2] == 1.0 and got.flat[3] == np.float32(1.0 + 2.0 ** -9)
+    assert got.flat[5] == 65504.0 and np.isinf(got.flat[6]) and 0 < got.flat[7] < 2.0 ** -14
+    u = np.random.default_rng(6).standard_normal((6, 8, 20)).astype(np.float32) * 100
+    g.set_velocity(2, 1, u)
+    assert np.array_equal(g.get_velocity(2, 1).view(np.uint32), _rne_f16_bits(u))
+    g.close()
+
+
 def test_bf16_store_is_round_to_nearest_even():
     g = smokesim.Simulation(19, 7, 5, storage=smokesim.STORAGE_BF16)
     x = np.random.default_rng(3).standard_normal((5, 7, 19)).astype(np.float32)
@@ -88,25 +123,39 @@ def _rne_bf16_bits(a):
     return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16).astype(np.uint32)
 
 
+def _rne_f16_bits(a):
+    # numpy's float32 -> float16 conversion rounds to nearest even, keeps
+    # subnormals and overflows to Inf: IEEE binary16.
+    with np.errstate(over="ignore"):  # values past 65504 round to Inf, as on the GPU
+        return np.ascontiguousarray(a, dtype=np.float32).astype(np.float16).astype(np.float32).view(np.uint32)
+
+
+# (storage, rounding of an FP32 value to the stored 16-bit format, as FP32 bits)
+HALF_FORMS = [(smokesim.STORAGE_BF16, _rne_bf16_bits), (smokesim.STORAGE_FP16, _rne_f16_bits)]
+SPLIT_FORMS = [(smokesim.STORAGE_BF16_VELOCITY, _rne_bf16_bits),
+               (smokesim.STORAGE_FP16_VELOCITY, _rne_f16_bits)]
+
+
+@pytest.mark.parametrize("storage,rnd", HALF_FORMS)
 @pytest.mark.parametrize("dims,seed,noise,dt", [((70, 21, 19), 50, 0.5, 0.05),
                                                  ((67, 18, 35), 51, 6.0, 0.3)])
-def test_bf16_advection_is_the_rounded_fp32_advection(dims, seed, noise, dt):
-    # BF16 fields widen to FP32 exactly and each output is rounded once, so
-    # the BF16 advection stages fed BF16-representable fields give, bit for
-    # bit, the BF16 rounding of the FP32 stages' outputs: the marched LDS ring,
+def test_half_advection_is_the_rounded_fp32_advection(dims, seed, noise, dt, storage, rnd):
+    # 16-bit fields widen to FP32 exactly and each output is rounded once, so
+    # the 16-bit advection stages fed representable fields give, bit for bit,
+    # the 16-bit rounding of the FP32 stages' outputs: the marched LDS ring,
     # its x-pair loads and the global fallback of back-traces longer than one
     # cell (the second case) included.
     sc = scenes.test_scene(dims, seed=seed, projection=smokesim.PROJ_JACOBI, iterations=4)
-    sims = [gpu_for(sc), gpu_for(sc, storage=smokesim.STORAGE_BF16)]
+    sims = [gpu_for(sc), gpu_for(sc, storage=storage)]
     for s in sims:
         s.run_stage(smokesim.STAGE_SCENE, dt)
     now = sims[0].index_now
     assert sims[1].index_now == now
     rng = np.random.default_rng(seed)
     face, cell = sims[0].face_shape, sims[0].cell_shape
-    vel = [_rne_bf16_bits(rng.uniform(-noise, noise, face).astype(np.float32)).view(np.float32)
+    vel = [rnd(rng.uniform(-noise, noise, face).astype(np.float32)).view(np.float32)
            for _ in range(3)]
-    rho = _rne_bf16_bits(rng.uniform(0.0, 1.0, cell).astype(np.float32)).view(np.float32)
+    rho = rnd(rng.uniform(0.0, 1.0, cell).astype(np.float32)).view(np.float32)
     for s in sims:
         for c in range(3):
             s.set_velocity(c, now, vel[c])
@@ -115,25 +164,26 @@ def test_bf16_advection_is_the_rounded_fp32_advection(dims, seed, noise, dt):
     past = 1 - now
     for c in range(3):
         a32, a16 = sims[0].get_velocity(c, past), sims[1].get_velocity(c, past)
-        assert np.array_equal(a16.view(np.uint32), _rne_bf16_bits(a32)), c
+        assert np.array_equal(a16.view(np.uint32), rnd(a32)), c
         sims[0].set_velocity(c, past, a16)  # the density stage reads the rounded faces
     for s in sims:
         s.run_stage(smokesim.STAGE_ADVECT_DENS, dt)
     d32, d16 = sims[0].get_density(past), sims[1].get_density(past)
-    assert np.array_equal(d16.view(np.uint32), _rne_bf16_bits(d32))
+    assert np.array_equal(d16.view(np.uint32), rnd(d32))
     for s in sims:
         s.close()
 
 
-def test_bf16_stages_are_the_rounded_fp32_stages():
+@pytest.mark.parametrize("storage,rnd", HALF_FORMS)
+def test_half_stages_are_the_rounded_fp32_stages(storage, rnd):
     # Stage by stage over two steps (sources, force, Jacobi projection,
-    # advection): with the FP32 simulation re-fed the BF16 simulation's
-    # fields before every stage, each BF16 stage's output fields are the BF16
-    # rounding of the FP32 stage's, bit for bit, and the FP32 pressure agrees
-    # exactly.
+    # advection): with the FP32 simulation re-fed the 16-bit simulation's
+    # fields before every stage, each 16-bit stage's output fields are the
+    # 16-bit rounding of the FP32 stage's, bit for bit, and the FP32 pressure
+    # agrees exactly.
     sc = scenes.test_scene((66, 20, 18), seed=52, projection=smokesim.PROJ_JACOBI, iterations=6,
                            noise=0.5)
-    g32, g16 = gpu_for(sc), gpu_for(sc, storage=smokesim.STORAGE_BF16)
+    g32, g16 = gpu_for(sc), gpu_for(sc, storage=storage)
     names = [("rho", h) for h in (0, 1)] + [(c, h) for c in range(3) for h in (0, 1)]
 
     def fields(g):
@@ -151,7 +201,7 @@ def test_bf16_stages_are_the_rounded_fp32_stages():
             g16.run_stage(stage, sc["dt"])
             f32, f16 = fields(g32), fields(g16)
             for k in names:
-                assert np.array_equal(f16[k].view(np.uint32), _rne_bf16_bits(f32[k])), (step, stage, k)
+                assert np.array_equal(f16[k].view(np.uint32), rnd(f32[k])), (step, stage, k)
             assert np.array_equal(g16.get_pressure().view(np.uint32),
                                   g32.get_pressure().view(np.uint32)), (step, stage)
     g32.close()
@@ -220,14 +270,15 @@ def test_mixed_form_at_256_stays_close_to_fp32():
     b.close()
 
 
-def test_bf16_velocity_stages_round_only_the_velocity():
-    # The split storage (FP32 density, BF16 u, v, w): stage by stage over two
-    # steps, with the FP32 simulation re-fed the split one's fields before
-    # every stage, velocity outputs are the BF16 rounding of the FP32 stage's
-    # and density outputs equal the FP32 stage's, bit for bit.
+@pytest.mark.parametrize("storage,rnd", SPLIT_FORMS)
+def test_half_velocity_stages_round_only_the_velocity(storage, rnd):
+    # The split storage (FP32 density, 16-bit u, v, w): stage by stage over
+    # two steps, with the FP32 simulation re-fed the split one's fields before
+    # every stage, velocity outputs are the 16-bit rounding of the FP32
+    # stage's and density outputs equal the FP32 stage's, bit for bit.
     sc = scenes.test_scene((66, 20, 18), seed=53, projection=smokesim.PROJ_JACOBI, iterations=6,
                            noise=0.5)
-    g32, gs = gpu_for(sc), gpu_for(sc, storage=smokesim.STORAGE_BF16_VELOCITY)
+    g32, gs = gpu_for(sc), gpu_for(sc, storage=storage)
     names = [("rho", h) for h in (0, 1)] + [(c, h) for c in range(3) for h in (0, 1)]
 
     def fields(g):
@@ -245,7 +296,7 @@ def test_bf16_velocity_stages_round_only_the_velocity():
             gs.run_stage(stage, sc["dt"])
             f32, fs = fields(g32), fields(gs)
             for k in names:
-                want = f32[k].view(np.uint32) if k[0] == "rho" else _rne_bf16_bits(f32[k])
+                want = f32[k].view(np.uint32) if k[0] == "rho" else rnd(f32[k])
                 assert np.array_equal(fs[k].view(np.uint32), want), (step, stage, k)
             assert np.array_equal(gs.get_pressure().view(np.uint32),
                                   g32.get_pressure().view(np.uint32)), (step, stage)
@@ -262,3 +313,27 @@ def test_bf16_velocity_within_tolerance(dims, seed, projection):
     assert err["rho"] <= 2e-3, err
     assert max(err["u"], err["v"], err["w"], err["pressure"]) <= 2e-2, err
     g.close()
+
+
+@pytest.mark.parametrize("storage", [smokesim.STORAGE_FP32, smokesim.STORAGE_BF16_VELOCITY])
+def test_refined_result_does_not_depend_on_the_launch_grouping(storage):
+    # 20 inner sweeps run as 10 launches of 2, 4+4+4+4+4, or 6+6+6+2: the
+    # same sweeps, the same bits.
+    sc = scenes.test_scene((97, 70, 45), seed=11, projection=smokesim.PROJ_JACOBI, iterations=60)
+    states = []
+    for t in (2, 4, 6):
+        g = gpu_for(sc, storage=storage)
+        g.set_jacobi_plan(sweeps_per_launch=t, tile=smokesim.TILE_32x32, zc=16)
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=20, iterations=60)
+        for _ in range(3):
+            g.step(sc["dt"], readback=False)
+        st = g.state()
+        st["pressure"] = g.get_pressure()
+        states.append((t, st, g.solve_info()))
+        g.close()
+    for t, st, info in states[1:]:
+        for k, v in st.items():
+            if isinstance(v, np.ndarray):
+                assert np.array_equal(np.ascontiguousarray(v).view(np.uint8),
+                                      np.ascontiguousarray(states[0][1][k]).view(np.uint8)), (t, k)
+        assert info == states[0][2], (t, info)

@@ -38,7 +38,8 @@ def random_view(seed):
     return box, dims, origin, d
 
 
-@pytest.mark.parametrize("steps", [1, 2, 20, 100, 200, 255, 256, 257, 1000, 4096])
+@pytest.mark.parametrize("steps", [1, 2, 20, 63, 64, 65, 100, 200, 255, 256, 257, 1000, 1023, 1024, 1025,
+                                   2500, 4096])
 def test_random_views_bitwise(steps):
     for seed in range(12):
         view = random_view(1000 * steps + seed)

@@ -16,6 +16,8 @@ pytestmark = pytest.mark.gpu
     (smokesim.PROJ_RBSOR, smokesim.STORAGE_FP32),
     (smokesim.PROJ_JACOBI, smokesim.STORAGE_FP32),
     (smokesim.PROJ_REFINED, smokesim.STORAGE_BF16),
+    (smokesim.PROJ_REFINED, smokesim.STORAGE_FP16),
+    (smokesim.PROJ_JACOBI, smokesim.STORAGE_FP16_VELOCITY),
 ])
 def test_restore_continues_bitwise(projection, storage):
     proj0 = smokesim.PROJ_JACOBI if projection == smokesim.PROJ_REFINED else projection
