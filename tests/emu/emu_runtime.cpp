@@ -5,6 +5,19 @@
 //  * "Device" allocations are plain heap blocks (AddressSanitizer bounds).
 //  * Work runs synchronously in submission order; a stream under capture
 //    records launches instead, and a graph launch replays them.
+//  * With EMU_DEFERRED set, stream work (launches, asynchronous copies,
+//    graph launches, event waits) is queued per stream and runs only when a
+//    synchronisation needs it -- stream or event synchronisation, an event
+//    wait reached by another stream, stream destruction, and hipFree /
+//    hipHostFree (which wait for the whole device, as HIP's do); the
+//    synchronous hipMemcpy runs at once without waiting for any stream (the
+//    library's streams are non-blocking).  Pending work of different streams
+//    runs newest stream first.  A host buffer changed or freed before the
+//    copy that reads it ran, a result read before its stream was
+//    synchronised, or a missing event wait between streams then gives wrong
+//    results or an AddressSanitizer report here.  With EMU_REVERSE_BLOCKS set,
+//    kernels that run their blocks one after another do so in reverse order,
+//    so results that depend on block order show up.
 //  * Kernels without barriers run their threads one after another; the
 //    fused-Jacobi kernels (the only ones with __syncthreads) run every thread
 //    of a block on its own host thread with a real barrier, one block at a
@@ -20,6 +33,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <deque>
 #include <functional>
 #include <memory>
 #include <set>
@@ -36,9 +50,13 @@ struct ihipStream_t {
     unsigned magic;
     bool capturing;
     std::vector<Op> rec;
+    std::deque<Op> queue;  // deferred work not yet run
+    unsigned long long enqueued = 0, done = 0;
 };
 struct ihipEvent_t {
     unsigned magic;
+    ihipStream_t* stream = nullptr;  // where it was last recorded
+    unsigned long long target = 0;   // that stream's work count at the record
 };
 struct ihipGraph {
     unsigned magic;
@@ -57,6 +75,10 @@ thread_local EmuIdx threadIdx, blockIdx, blockDim, gridDim;
 namespace {
 
 std::set<void*> g_device;
+const bool g_deferred = getenv("EMU_DEFERRED") != nullptr;
+const bool g_reverse_blocks = getenv("EMU_REVERSE_BLOCKS") != nullptr;
+ihipStream_t g_null_stream{0x5354524dU, false, {}, {}, 0, 0};
+std::vector<ihipStream_t*> g_streams;  // live streams, in creation order (null stream first)
 std::barrier<>* g_barrier = nullptr;
 // One barrier per wave of 64 threads for lane exchanges (DPP, shuffles): a
 // wave may skip work that other waves of the block do (wave-uniform
@@ -76,6 +98,49 @@ void check(hipStream_t s)
 void check(hipEvent_t e)
 {
     if (!e || e->magic != kEvent) die("use of an invalid event");
+}
+ihipStream_t* S(hipStream_t s) { return s ? s : &g_null_stream; }
+
+// Runs queued work of `s` until `upto` items of it have run.  Work may drain
+// other streams (event waits).
+void drain(ihipStream_t* s, unsigned long long upto)
+{
+    while (s->done < upto) {
+        if (s->queue.empty()) die("drain past the queued work");
+        Op op = std::move(s->queue.front());
+        s->queue.pop_front();
+        op();
+        s->done += 1;
+    }
+}
+void drain_all()
+{
+    if (g_null_stream.done < g_null_stream.enqueued) drain(&g_null_stream, g_null_stream.enqueued);
+    for (;;) {
+        bool any = false;
+        for (size_t i = g_streams.size(); i-- > 0;) {
+            ihipStream_t* s = g_streams[i];
+            if (s->done < s->enqueued) {
+                drain(s, s->enqueued);
+                any = true;
+                break;  // the list may have changed
+            }
+        }
+        if (!any) break;
+    }
+    if (g_null_stream.done < g_null_stream.enqueued) drain(&g_null_stream, g_null_stream.enqueued);
+}
+void enqueue(hipStream_t st, Op op)
+{
+    ihipStream_t* s = S(st);
+    if (!g_deferred) {
+        op();
+        s->enqueued += 1;
+        s->done += 1;
+        return;
+    }
+    s->queue.push_back(std::move(op));
+    s->enqueued += 1;
 }
 
 // Lane exchanges use two buffers in turn, so one wave barrier per exchange
@@ -149,15 +214,16 @@ void run(const char* name, const std::function<void()>& fn, dim3 grid, dim3 bloc
     if (!threaded) {
         blockDim = bd;
         gridDim = gd;
-        for (unsigned bz = 0; bz < grid.z; ++bz)
-            for (unsigned by = 0; by < grid.y; ++by)
-                for (unsigned bx = 0; bx < grid.x; ++bx) {
-                    blockIdx = {bx, by, bz};
-                    for (unsigned t = 0; t < nt; ++t) {
-                        set_thread(t);
-                        fn();
-                    }
-                }
+        const unsigned long long nb = (unsigned long long)grid.x * grid.y * grid.z;
+        for (unsigned long long k = 0; k < nb; ++k) {
+            const unsigned long long b = g_reverse_blocks ? nb - 1 - k : k;
+            blockIdx = {(unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
+                        (unsigned)(b / ((unsigned long long)grid.x * grid.y))};
+            for (unsigned t = 0; t < nt; ++t) {
+                set_thread(t);
+                fn();
+            }
+        }
         return;
     }
     std::barrier<> bar((std::ptrdiff_t)nt);
@@ -232,7 +298,7 @@ void emu_launch(const char* name, std::function<void()> fn, dim3 grid, dim3 bloc
         st->rec.push_back([=]() { run(name, fn, grid, block); });
         return;
     }
-    run(name, fn, grid, block);
+    enqueue(st, [=]() { run(name, fn, grid, block); });
 }
 
 extern "C" {
@@ -246,6 +312,7 @@ hipError_t hipMalloc(void** p, size_t n)
 hipError_t hipFree(void* p)
 {
     if (!p) return hipSuccess;
+    drain_all();
     if (!g_device.erase(p)) die("hipFree of a pointer hipMalloc did not return");
     free(p);
     return hipSuccess;
@@ -257,6 +324,7 @@ hipError_t hipHostMalloc(void** p, size_t n, unsigned)
 }
 hipError_t hipHostFree(void* p)
 {
+    drain_all();
     free(p);
     return hipSuccess;
 }
@@ -269,7 +337,7 @@ hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipSt
 {
     check(st);
     if (st && st->capturing) die("copy recorded into a graph");
-    memcpy(d, s, n);
+    enqueue(st, [=]() { memcpy(d, s, n); });
     return hipSuccess;
 }
 hipError_t hipGetDevice(int* d)
@@ -300,26 +368,37 @@ hipError_t hipGetDevicePropertiesR0600(hipDeviceProp_tR0600* p, int)
 const char* hipGetErrorString(hipError_t) { return "emulator error"; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned)
 {
-    *s = new ihipStream_t{kStream, false, {}};
+    *s = new ihipStream_t{kStream, false, {}, {}, 0, 0};
+    g_streams.push_back(*s);
     return hipSuccess;
 }
 hipError_t hipStreamDestroy(hipStream_t s)
 {
     check(s);
+    if (!s) return hipErrorInvalidValue;
+    drain(s, s->enqueued);
+    g_streams.erase(std::remove(g_streams.begin(), g_streams.end(), s), g_streams.end());
     s->magic = 0;
-    delete s;
+    // Deferred event waits of other streams may still name it (drained, so
+    // they return at once): kept allocated in that mode.
+    if (!g_deferred) delete s;
     return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t s)
 {
     check(s);
     if (s && s->capturing) die("synchronize on a capturing stream");
+    drain(S(s), S(s)->enqueued);
     return hipSuccess;
 }
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned)
 {
     check(s);
     check(e);
+    if (s && s->capturing) die("event wait recorded into a graph");
+    ihipStream_t* es = e->stream;
+    const unsigned long long t = e->target;
+    if (es) enqueue(s, [es, t]() { drain(es, t); });
     return hipSuccess;
 }
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode)
@@ -346,7 +425,7 @@ hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s)
 {
     check(s);
     if (!e || e->magic != kExec) die("launch of an invalid graph");
-    for (Op& op : e->ops) op();
+    for (Op& op : e->ops) enqueue(s, op);
     return hipSuccess;
 }
 hipError_t hipGraphExecDestroy(hipGraphExec_t e)
@@ -384,22 +463,27 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s)
 {
     check(e);
     check(s);
+    if (s && s->capturing) return hipSuccess;  // not part of the recorded work
+    e->stream = S(s);
+    e->target = S(s)->enqueued;
     return hipSuccess;
 }
 hipError_t hipEventSynchronize(hipEvent_t e)
 {
     check(e);
+    if (e->stream) drain(e->stream, e->target);
     return hipSuccess;
 }
 hipError_t hipEventQuery(hipEvent_t e)
 {
     check(e);
-    return hipSuccess;
+    return !e->stream || e->stream->done >= e->target ? hipSuccess : hipErrorNotReady;
 }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b)
 {
     check(a);
     check(b);
+    if (hipEventQuery(a) != hipSuccess || hipEventQuery(b) != hipSuccess) return hipErrorNotReady;
     *ms = 1.0f;
     return hipSuccess;
 }

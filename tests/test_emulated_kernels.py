@@ -2,7 +2,11 @@
 under AddressSanitizer (tests/emu), run small single-simulation, Jacobi-plan,
 batch (with graph replay), serving-loop and read-back cases and must give the
 FP32 oracle's bits.  This checks every kernel index against its allocation on
-the CPU and covers the serving loop's logic without a GPU."""
+the CPU and covers the serving loop's logic without a GPU.  The cases that use
+several streams run again with stream work deferred until a synchronisation
+needs it (newest stream first) and blocks in reverse order: a missing wait
+between streams, a host buffer reused before its copy ran, or a result that
+depends on block order then fails here."""
 import os
 import subprocess
 import sys
@@ -41,6 +45,16 @@ def emu_lib(tmp_path_factory):
 def test_emulated_case_matches_oracle(emu_lib, case):
     lib, asan = emu_lib
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0")
+    r = subprocess.run([sys.executable, os.path.join(EMU, "emu_cases.py"), lib, case], env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and f"ok {case}" in r.stdout, (r.stdout[-4000:] + r.stderr[-8000:])
+
+
+@pytest.mark.parametrize("case", ["single", "batch", "server", "readback", "norms"])
+def test_emulated_case_with_deferred_streams(emu_lib, case):
+    lib, asan = emu_lib
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
+               EMU_DEFERRED="1", EMU_REVERSE_BLOCKS="1")
     r = subprocess.run([sys.executable, os.path.join(EMU, "emu_cases.py"), lib, case], env=env,
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and f"ok {case}" in r.stdout, (r.stdout[-4000:] + r.stderr[-8000:])
