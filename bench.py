@@ -286,7 +286,42 @@ def error_check(sc, steps, mixed_iterations):
     return out
 
 
-def time_form(args, sc, storage, extra=None, plan=None):
+def time_readback(sim, dt, n):
+    """Steps with the density brought to host memory after every step, as a
+    renderer shows it (host clock, ms per step): the reference's blocking
+    form (smoke_step: the step, then the copy into a plain host array, then
+    return) and the double-buffered asynchronous read-back
+    (smoke_readback_begin after each step; the previous step's density is
+    waited for, so its PCIe copy overlaps the next step), FP32 and FP16."""
+    import ctypes
+    import numpy as np
+    import smokesim
+    lib = sim.lib
+    host = np.empty(sim.cell_shape, np.float32)
+    hp = host.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    out = {"steps": n}
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        smokesim._check(lib.smoke_step(sim.h, ctypes.c_float(dt), hp))
+    out["blocking_f32_ms_per_step"] = round(1000.0 * (time.perf_counter() - t0) / n, 4)
+    ptr, step = ctypes.c_void_p(), ctypes.c_longlong()
+    for fmt, name in ((smokesim.READBACK_F32, "async_f32_ms_per_step"),
+                      (smokesim.READBACK_F16, "async_f16_ms_per_step")):
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for k in range(n):
+            smokesim._check(lib.smoke_step_async(sim.h, ctypes.c_float(dt)))
+            smokesim._check(lib.smoke_readback_begin(sim.h, fmt))
+            if k:
+                smokesim._check(lib.smoke_readback_wait(sim.h, ctypes.byref(ptr), ctypes.byref(step)))
+        smokesim._check(lib.smoke_readback_wait(sim.h, ctypes.byref(ptr), ctypes.byref(step)))
+        sim.synchronize()
+        out[name] = round(1000.0 * (time.perf_counter() - t0) / n, 4)
+    return out
+
+
+def time_form(args, sc, storage, extra=None, plan=None, readback=False):
     """Creates the simulation of `sc` with the given field storage (and extra
     parameters), applies the Jacobi plan (`plan`, --plan, or the autotuner's
     for the Jacobi projection), runs the warm-up steps, then times
@@ -331,11 +366,12 @@ def time_form(args, sc, storage, extra=None, plan=None):
         raise SystemExit("non-finite density after the timed steps")
     norms = sim.step_norms()
     p = sim.get_params()
+    rb = time_readback(sim, dt, min(args.steps, 10)) if readback else None
     res = {"wall": wall, "timing": timing, "step_ms": step_ms, "sweep_ms": sweep_ms,
            "sweep_launches": sweep_launches, "params": p, "plan": sim.get_jacobi_plan(),
            "autotune": tuned, "density_sum": float(dens.astype(np.float64).sum()),
            "norms": {k: norms[k] for k in ("div_l2", "rhs_l2", "cfl", "nonfinite")},
-           "solve": sim.solve_info() if p["projection"] == 3 else None}
+           "solve": sim.solve_info() if p["projection"] == 3 else None, "readback": rb}
     sim.close()
     log(f"timed: {args.steps} steps in {wall:.3f} s; density sum {res['density_sum']:.3f}")
     return res
@@ -353,7 +389,7 @@ def run_single(args, sc):
     fp32_iterations = sc["params"]["iterations"]
     log("workload:", describe(sc))
     storage = MIXED_STORAGE if mixed else smokesim.STORAGE_FP32
-    r = time_form(args, sc, storage, dict(MIXED) if mixed else None)
+    r = time_form(args, sc, storage, dict(MIXED) if mixed else None, readback=not mixed)
     ms_per_step = 1000.0 * r["wall"] / args.steps
     p = r["params"]
     timing = r["timing"]
@@ -463,6 +499,7 @@ def run_single(args, sc):
                                         "frac": round(step_eff / HBM_PEAK_GBS, 4),
                                         "model": "unfused streaming model (121 + 13 K B/cell)"}},
         "roofline": roofline,
+        "readback": r["readback"],
         "cpu_baseline": None,
     }
     if not mixed and not args.no_mixed and p["projection"] == 2:

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 
@@ -127,10 +128,13 @@ void emu_launch(const char* name, std::function<void()> fn, dim3 grid, dim3 bloc
 #define hipLaunchKernelGGL(K, G, B, SH, ST, ...) \
     emu_launch(#K, [=]() { K(__VA_ARGS__); }, dim3(G), dim3(B), ST)
 
+// Resident workgroups per CU; EMU_OCCUPANCY overrides it, so the persistent
+// grids of the batched kernels take other sizes.
 template <class F>
 hipError_t emu_occupancy(int* n, F, int threads, size_t)
 {
-    *n = threads >= 1024 ? 2 : 2048 / threads;
+    const char* env = getenv("EMU_OCCUPANCY");
+    *n = env ? atoi(env) : (threads >= 1024 ? 2 : 2048 / threads);
     return hipSuccess;
 }
 #define hipOccupancyMaxActiveBlocksPerMultiprocessor emu_occupancy
