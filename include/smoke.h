@@ -145,7 +145,7 @@ smoke_status smoke_create(unsigned nx, unsigned ny, unsigned nz, const float* de
                           smoke_sim** out);
 
 /* smoke_create with a field storage (smoke_storage).  No reference
- * counterpart: the mixed-precision form's BF16 fields. */
+ * counterpart: the mixed-precision forms' 16-bit fields. */
 smoke_status smoke_create_ex(unsigned nx, unsigned ny, unsigned nz, const float* density,
                              int storage, smoke_sim** out);
 int smoke_get_storage(const smoke_sim* sim);
