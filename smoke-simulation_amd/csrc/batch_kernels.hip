@@ -146,7 +146,7 @@ __global__ __launch_bounds__(BX * BY) void kb_adv_d(const SimDesc* __restrict__ 
 
 // Fused Jacobi sweeps over every tile of every simulation.
 template <int TX, int TY, int T, bool DAMPED>
-__global__ __launch_bounds__(TX * TY) void kb_jacobi(const SimDesc* __restrict__ sims,
+__global__ __launch_bounds__(TX * TY, 8) void kb_jacobi(const SimDesc* __restrict__ sims,
                                                      const StageTable* __restrict__ tab, int nsims,
                                                      int lp, int zc, int zero_in, float omega)
 {

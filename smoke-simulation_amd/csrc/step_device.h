@@ -387,14 +387,25 @@ static_assert(fused_slack_planes(2) <= kFusedSlackPlanes && fused_slack_planes(3
 // from 2^31) read 0, so the loads need no validity selects.  Requires
 // zc <= fused_max_zc(plane, T).  All threads of the block must call this with
 // the same tile (the barriers are block-wide).
-template <int TX, int TY, int T, bool DAMPED>
-__device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __restrict__ pin,
-                                                  float* __restrict__ pout,
-                                                  const float* __restrict__ d,
-                                                  const uint8_t* __restrict__ flags, int zc,
-                                                  int zero_in, float omega, int tile_x, int tile_y,
-                                                  int tile_z, FusedLds<TX, TY, T>& sh)
+//
+// Level t is valid only on tile rows [t, TY - t) (a row needs its two
+// neighbour rows of level t-1), and valid rows read only valid rows.  A wave
+// holds 64 / TX whole rows, so a wave none of whose rows is valid at level t
+// skips that level and every level above: it runs jacobi_fused_levels with NL
+// < T computed levels (a wave-uniform choice).  Every variant runs the same
+// plane loop with the same barriers, stores its levels 0..NL into LDS for the
+// rows above and below, and writes no output (written rows lie in waves with
+// NL = T).
+template <int TX, int TY, int T, bool DAMPED, int NL>
+__device__ __forceinline__ void jacobi_fused_levels(const Grid& g, const float* __restrict__ pin,
+                                                    float* __restrict__ pout,
+                                                    const float* __restrict__ d,
+                                                    const uint8_t* __restrict__ flags, int zc,
+                                                    int zero_in, float omega, int tile_x,
+                                                    int tile_y, int tile_z,
+                                                    FusedLds<TX, TY, T>& sh)
 {
+    static_assert(NL >= 0 && NL <= T, "levels of a wave");
     const int tx = threadIdx.x, ty = threadIdx.y;
     constexpr int OX = TX - 2 * T, OY = TY - 2 * T;
     const int x = tile_x * OX + tx - T + 1;
@@ -472,7 +483,7 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
         float newv[T + 1];
         newv[0] = p0;
 #pragma unroll
-        for (int t = 1; t <= T; ++t) {
+        for (int t = 1; t <= NL; ++t) {
             // x neighbours of level t-1 at plane k-t from the adjacent lanes
             // (w[t-1][1] holds that plane for every lane), y neighbours from LDS.
             const float sum = sum6(lane_prev(w[t - 1][1]), lane_next(w[t - 1][1]),
@@ -488,19 +499,62 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
                 v = G;
             }
             newv[t] = v;
-            if (t < T) {
+            if (t < NL) {
                 w[t][0] = w[t][1];
                 w[t][1] = w[t][2];
                 w[t][2] = v;
             }
         }
         const int zo = k - T;
-        if (writes && zo >= z0 && zo < z1)
-            buf_st_f32(ro, lane_b + (uint32_t)(zo - plo) * plane * 4u, newv[T]);
+        if (NL == T && writes && zo >= z0 && zo < z1)
+            buf_st_f32(ro, lane_b + (uint32_t)(zo - plo) * plane * 4u, newv[NL]);
 #pragma unroll
-        for (int t = 0; t < T; ++t) sh.v[t][cur][ty + 1][tx + 1] = newv[t];
+        for (int t = 0; t < T && t <= NL; ++t) sh.v[t][cur][ty + 1][tx + 1] = newv[t];
         __syncthreads();
     }
+}
+
+// The wave's level count NL (see jacobi_fused_levels) as a template argument.
+template <int TX, int TY, int T, bool DAMPED, int K>
+__device__ __forceinline__ void jacobi_fused_dispatch(int nl, const Grid& g,
+                                                      const float* __restrict__ pin,
+                                                      float* __restrict__ pout,
+                                                      const float* __restrict__ d,
+                                                      const uint8_t* __restrict__ flags, int zc,
+                                                      int zero_in, float omega, int tile_x,
+                                                      int tile_y, int tile_z,
+                                                      FusedLds<TX, TY, T>& sh)
+{
+    if constexpr (K == 0) {
+        jacobi_fused_levels<TX, TY, T, DAMPED, 0>(g, pin, pout, d, flags, zc, zero_in, omega,
+                                                  tile_x, tile_y, tile_z, sh);
+    } else {
+        if (nl == K)
+            jacobi_fused_levels<TX, TY, T, DAMPED, K>(g, pin, pout, d, flags, zc, zero_in, omega,
+                                                      tile_x, tile_y, tile_z, sh);
+        else
+            jacobi_fused_dispatch<TX, TY, T, DAMPED, K - 1>(nl, g, pin, pout, d, flags, zc,
+                                                            zero_in, omega, tile_x, tile_y,
+                                                            tile_z, sh);
+    }
+}
+
+template <int TX, int TY, int T, bool DAMPED>
+__device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __restrict__ pin,
+                                                  float* __restrict__ pout,
+                                                  const float* __restrict__ d,
+                                                  const uint8_t* __restrict__ flags, int zc,
+                                                  int zero_in, float omega, int tile_x, int tile_y,
+                                                  int tile_z, FusedLds<TX, TY, T>& sh)
+{
+    static_assert(64 % TX == 0, "a wave holds whole tile rows");
+    constexpr int RPW = 64 / TX;  // rows per wave
+    // First row of this thread's wave (uniform across the wave) and the
+    // highest level with a valid row in the wave: t <= last row, t <= TY-1-first row.
+    const int r0 = __builtin_amdgcn_readfirstlane((int)threadIdx.y / RPW * RPW);
+    const int nl = min(T, min(r0 + RPW - 1, TY - 1 - r0));
+    jacobi_fused_dispatch<TX, TY, T, DAMPED, T>(nl, g, pin, pout, d, flags, zc, zero_in, omega,
+                                                tile_x, tile_y, tile_z, sh);
 }
 
 // u -= grad P on every face whose two cells are fluid.  Faces that touch no

@@ -332,7 +332,7 @@ __device__ __forceinline__ int xcd_tile_order()
 
 // T fused Jacobi sweeps, one tile per block (dev::jacobi_fused_tile).
 template <int TX, int TY, int T, bool DAMPED>
-__global__ __launch_bounds__(TX * TY) void k_jacobi_fused(
+__global__ __launch_bounds__(TX * TY, 8) void k_jacobi_fused(
     Grid g, const float* __restrict__ pin, float* __restrict__ pout, const float* __restrict__ d,
     const uint8_t* __restrict__ flags, int zc, int zero_in, float omega)
 {

@@ -46,6 +46,10 @@ inline void __threadfence_system() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 // constraint does not exist on the host).
 #define SMK_OPAQUE_F32(x) ((void)(x))
 
+// Wave-uniform values only (the kernels apply it to values that already are):
+// the value itself.
+#define __builtin_amdgcn_readfirstlane(x) (x)
+
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline unsigned __float_as_uint(float f)
 {
