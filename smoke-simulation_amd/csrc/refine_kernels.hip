@@ -118,7 +118,7 @@ __global__ __launch_bounds__(BX * BY) void k_ref_norm0(Grid g, const float* __re
 // stores issued last, so the loads stay in flight together.  Once `done` is
 // set the pass only copies x_old.
 template <bool XZERO, bool LAST>
-__global__ __launch_bounds__(BX * BY) void k_ref_resid(Grid g, const float* __restrict__ xo,
+__global__ __launch_bounds__(BX * BY, 8) void k_ref_resid(Grid g, const float* __restrict__ xo,
                                                        float* __restrict__ xn,
                                                        const float* __restrict__ d,
                                                        const uint8_t* __restrict__ flags,
@@ -307,7 +307,7 @@ struct HalfLds {
 // e starts at 0 and the scaled right-hand side inv*r*s is formed from the FP32
 // r and written for the later launches of the refinement.
 template <int TXP, int TY, int T, bool FIRST>
-__global__ __launch_bounds__(TXP * TY) void k_ref_sweeps(Grid g, const uint32_t* __restrict__ ein,
+__global__ __launch_bounds__(TXP * TY, 8) void k_ref_sweeps(Grid g, const uint32_t* __restrict__ ein,
                                                          uint32_t* __restrict__ eout,
                                                          const float* __restrict__ q,
                                                          uint32_t* __restrict__ qi,
