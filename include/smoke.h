@@ -184,7 +184,14 @@ smoke_status smoke_get_params(const smoke_sim* sim, smoke_params* out);
  * Every plan gives bit-identical results; only the speed differs.  No
  * reference counterpart (the reference launches one kernel per half-sweep,
  * smokeSimulation.cu:797-801). */
-typedef enum smoke_tile { SMOKE_TILE_32x32 = 0, SMOKE_TILE_64x16 = 1 } smoke_tile;
+/* Tiles 64x32 and 32x64 give each thread two rows of its column (fewer halo
+ * cells per written cell, one 1024-thread block per CU instead of two). */
+typedef enum smoke_tile {
+    SMOKE_TILE_32x32 = 0,
+    SMOKE_TILE_64x16 = 1,
+    SMOKE_TILE_64x32 = 2,
+    SMOKE_TILE_32x64 = 3
+} smoke_tile;
 typedef struct smoke_jacobi_plan {
     int sweeps_per_launch; /* 1..6 (default 4) */
     int tile;              /* smoke_tile (default 32x32) */
@@ -195,8 +202,9 @@ smoke_status smoke_get_jacobi_plan(const smoke_sim* sim, smoke_jacobi_plan* out)
 
 /* Autotuner: times the projection's sweep sequence (the simulation's current
  * iteration count, or 100 if that is 0) on this simulation's buffers for
- * every candidate plan -- 2..6 sweeps per launch x both tiles x z-chunks of
- * 32, 64 and 128 planes -- and keeps the fastest.  Results are identical for
+ * every candidate plan -- 2..6 sweeps per launch x the four tiles x z-chunks
+ * of 32, 64 and 128 planes and of the interior's nz-2 planes split into 1..4
+ * equal chunks -- and keeps the fastest.  Results are identical for
  * every plan, so tuning never changes answers.  Resets the pressure kept for
  * warm starts to zero.  `chosen` (may be NULL) receives the plan and
  * `ms_per_sweep` (may be NULL) its measured time per sweep. */
@@ -384,8 +392,9 @@ smoke_status smoke_batch_step(smoke_batch* batch, int use_graph);
 smoke_status smoke_batch_synchronize(smoke_batch* batch);
 smoke_status smoke_batch_set_jacobi_plan(smoke_batch* batch, const smoke_jacobi_plan* plan);
 smoke_status smoke_batch_get_jacobi_plan(const smoke_batch* batch, smoke_jacobi_plan* out);
-/* Autotuner of the batched sweep sequence (same candidates as
- * smoke_autotune); invalidates a captured graph. */
+/* Autotuner of the batched sweep sequence (2..6 sweeps per launch x the four
+ * tiles x z-chunks of 32, 64, 128 and 256 planes); invalidates a captured
+ * graph. */
 smoke_status smoke_batch_autotune(smoke_batch* batch, smoke_jacobi_plan* chosen, double* ms_per_sweep);
 /* Step latencies (ms, HIP events around each step on the batch's stream) of
  * the last `cap` steps, oldest first. */

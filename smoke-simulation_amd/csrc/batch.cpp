@@ -90,8 +90,7 @@ void fill_table(smk::StageTable& t, const std::vector<smoke_sim*>& sims, int kin
         int gx, gy, gz;
         if (kind == kTabJacobi) {
             const int T = plan.sweeps_per_launch;
-            const int tx = plan.tile == smk::JacobiPlan::TILE_32x32 ? 32 : 64;
-            const int ty = plan.tile == smk::JacobiPlan::TILE_32x32 ? 32 : 16;
+            const int tx = smk::tile_width(plan.tile), ty = smk::tile_height(plan.tile);
             gx = ceil_div(g.nx - 2, tx - 2 * T);
             gy = ceil_div(g.ny - 2, ty - 2 * T);
             gz = ceil_div(g.nz - 2, plan.zc);
@@ -464,7 +463,7 @@ smoke_status smoke_batch_set_jacobi_plan(smoke_batch* b, const smoke_jacobi_plan
     if (!b || !plan) return fail(SMOKE_ERR_INVALID, "batch_set_jacobi_plan: NULL");
     if (b->broken) return fail(SMOKE_ERR_STATE, "batch_set_jacobi_plan: a simulation was destroyed");
     if (plan->sweeps_per_launch < 1 || plan->sweeps_per_launch > 6 || plan->zc < 1 ||
-        (plan->tile != SMOKE_TILE_32x32 && plan->tile != SMOKE_TILE_64x16))
+        plan->tile < SMOKE_TILE_32x32 || plan->tile > SMOKE_TILE_32x64)
         return fail(SMOKE_ERR_INVALID, "batch_set_jacobi_plan: invalid plan");
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->plan.sweeps_per_launch = plan->sweeps_per_launch;
@@ -495,8 +494,9 @@ smoke_status smoke_batch_autotune(smoke_batch* b, smoke_jacobi_plan* chosen, dou
     HIP_TRY(hipEventCreate(&ev[1]));
     smk::JacobiPlan best = b->plan;
     double best_ms = -1.0;
-    const int tiles[2] = {smk::JacobiPlan::TILE_32x32, smk::JacobiPlan::TILE_64x16};
-    const int zcs[3] = {32, 64, 128};
+    const int tiles[4] = {smk::JacobiPlan::TILE_32x32, smk::JacobiPlan::TILE_64x16,
+                          smk::JacobiPlan::TILE_64x32, smk::JacobiPlan::TILE_32x64};
+    const int zcs[4] = {32, 64, 128, 256};
     hipError_t e = hipSuccess;
     for (int t = 2; t <= 6 && e == hipSuccess; ++t)
         for (int tile : tiles)

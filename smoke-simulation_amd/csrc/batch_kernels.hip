@@ -144,9 +144,10 @@ __global__ __launch_bounds__(BX * BY) void kb_adv_d(const SimDesc* __restrict__ 
     }
 }
 
-// Fused Jacobi sweeps over every tile of every simulation.
-template <int TX, int TY, int T, bool DAMPED>
-__global__ __launch_bounds__(TX * TY, 8) void kb_jacobi(const SimDesc* __restrict__ sims,
+// Fused Jacobi sweeps over every tile of every simulation (TX x TY tiles,
+// R tile rows per thread).
+template <int TX, int TY, int T, bool DAMPED, int R>
+__global__ __launch_bounds__(TX * TY / R, 8 / R) void kb_jacobi(const SimDesc* __restrict__ sims,
                                                      const StageTable* __restrict__ tab, int nsims,
                                                      int lp, int zc, int zero_in, float omega)
 {
@@ -159,9 +160,9 @@ __global__ __launch_bounds__(TX * TY, 8) void kb_jacobi(const SimDesc* __restric
         const int gx = tab->gx[si], gy = tab->gy[si];
         const SimDesc& s = sims[si];
         const int in = s.pcur ^ lp;
-        dev::jacobi_fused_tile<TX, TY, T, DAMPED>(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, zc,
-                                                  zero_in, omega, lb % gx, (lb / gx) % gy,
-                                                  lb / (gx * gy), sh);
+        dev::jacobi_fused_tile<TX, TY, T, DAMPED, R>(s.g, s.P[in], s.P[in ^ 1], s.d, s.flags, zc,
+                                                     zero_in, omega, lb % gx, (lb / gx) % gy,
+                                                     lb / (gx * gy), sh);
     }
 }
 
@@ -197,29 +198,30 @@ __global__ void kb_flip(SimDesc* __restrict__ sims, int nsims, int pflip)
     sims[i].pcur ^= pflip;
 }
 
-template <int TX, int TY, int T>
+template <int TX, int TY, int R, int T>
 hipError_t launch_bj(const SimDesc* sims, const StageTable* tab, int nsims, int grid, int lp,
                      int zc, int zero_in, float omega, int plain, hipStream_t st)
 {
+    const dim3 block(TX, TY / R);
     if (plain)
-        hipLaunchKernelGGL((kb_jacobi<TX, TY, T, false>), dim3((unsigned)grid), dim3(TX, TY), 0, st,
+        hipLaunchKernelGGL((kb_jacobi<TX, TY, T, false, R>), dim3((unsigned)grid), block, 0, st,
                            sims, tab, nsims, lp, zc, zero_in, omega);
     else
-        hipLaunchKernelGGL((kb_jacobi<TX, TY, T, true>), dim3((unsigned)grid), dim3(TX, TY), 0, st,
+        hipLaunchKernelGGL((kb_jacobi<TX, TY, T, true, R>), dim3((unsigned)grid), block, 0, st,
                            sims, tab, nsims, lp, zc, zero_in, omega);
     return hipGetLastError();
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int R>
 hipError_t launch_bj_t(int t, const SimDesc* sims, const StageTable* tab, int nsims, int grid,
                        int lp, int zc, int zero_in, float omega, int plain, hipStream_t st)
 {
     switch (t) {
-    case 2: return launch_bj<TX, TY, 2>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
-    case 3: return launch_bj<TX, TY, 3>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
-    case 4: return launch_bj<TX, TY, 4>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
-    case 5: return launch_bj<TX, TY, 5>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
-    case 6: return launch_bj<TX, TY, 6>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 2: return launch_bj<TX, TY, R, 2>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 3: return launch_bj<TX, TY, R, 3>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 4: return launch_bj<TX, TY, R, 4>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 5: return launch_bj<TX, TY, R, 5>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
+    case 6: return launch_bj<TX, TY, R, 6>(sims, tab, nsims, grid, lp, zc, zero_in, omega, plain, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -231,6 +233,20 @@ int occupancy(K kernel, int threads)
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1)
         n = 1;
     return n;
+}
+
+// Resident blocks per CU of the plain T-sweep kernel of one tile.
+template <int TX, int TY, int R>
+int bj_occupancy(int t)
+{
+    constexpr int threads = TX * TY / R;
+    switch (t) {
+    case 2: return occupancy(kb_jacobi<TX, TY, 2, false, R>, threads);
+    case 3: return occupancy(kb_jacobi<TX, TY, 3, false, R>, threads);
+    case 4: return occupancy(kb_jacobi<TX, TY, 4, false, R>, threads);
+    case 5: return occupancy(kb_jacobi<TX, TY, 5, false, R>, threads);
+    default: return occupancy(kb_jacobi<TX, TY, 6, false, R>, threads);
+    }
 }
 
 }  // namespace
@@ -270,11 +286,17 @@ hipError_t batch_jacobi(const SimDesc* sims, const StageTable* tab, int nsims, i
     if (plan.zc < 1) return hipErrorInvalidValue;
     switch (plan.tile) {
     case JacobiPlan::TILE_32x32:
-        return launch_bj_t<32, 32>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
-                                   plain, st);
+        return launch_bj_t<32, 32, 1>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
+                                      plain, st);
     case JacobiPlan::TILE_64x16:
-        return launch_bj_t<64, 16>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
-                                   plain, st);
+        return launch_bj_t<64, 16, 1>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
+                                      plain, st);
+    case JacobiPlan::TILE_64x32:
+        return launch_bj_t<64, 32, 2>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
+                                      plain, st);
+    case JacobiPlan::TILE_32x64:
+        return launch_bj_t<32, 64, 2>(sweeps, sims, tab, nsims, grid, lp, plan.zc, zero_in, omega,
+                                      plain, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -311,13 +333,11 @@ int batch_advect_density_blocks_per_cu() { return occupancy(kb_adv_d, BX * BY); 
 int batch_jacobi_blocks_per_cu(const JacobiPlan& plan, int sweeps)
 {
     if (sweeps == 1) return occupancy(kb_jacobi1, BX * BY);
-    const bool big = plan.tile == JacobiPlan::TILE_32x32;
-    switch (sweeps) {
-    case 2: return big ? occupancy(kb_jacobi<32, 32, 2, false>, 1024) : occupancy(kb_jacobi<64, 16, 2, false>, 1024);
-    case 3: return big ? occupancy(kb_jacobi<32, 32, 3, false>, 1024) : occupancy(kb_jacobi<64, 16, 3, false>, 1024);
-    case 4: return big ? occupancy(kb_jacobi<32, 32, 4, false>, 1024) : occupancy(kb_jacobi<64, 16, 4, false>, 1024);
-    case 5: return big ? occupancy(kb_jacobi<32, 32, 5, false>, 1024) : occupancy(kb_jacobi<64, 16, 5, false>, 1024);
-    default: return big ? occupancy(kb_jacobi<32, 32, 6, false>, 1024) : occupancy(kb_jacobi<64, 16, 6, false>, 1024);
+    switch (plan.tile) {
+    case JacobiPlan::TILE_64x16: return bj_occupancy<64, 16, 1>(sweeps);
+    case JacobiPlan::TILE_64x32: return bj_occupancy<64, 32, 2>(sweeps);
+    case JacobiPlan::TILE_32x64: return bj_occupancy<32, 64, 2>(sweeps);
+    default: return bj_occupancy<32, 32, 1>(sweeps);
     }
 }
 

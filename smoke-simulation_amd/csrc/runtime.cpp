@@ -977,7 +977,7 @@ smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
     if (h.n_objects < 0 || h.n_objects > 2 * SMOKE_MAX_OBJECTS || h.arena_bytes != arena_bytes(s->g, s->storage) ||
         bytes < snapshot_size(s, (size_t)h.n_objects) || (h.now != 0 && h.now != 1) ||
         (h.pcur != 0 && h.pcur != 1) || h.plan[0] < 1 || h.plan[0] > 6 ||
-        (h.plan[1] != SMOKE_TILE_32x32 && h.plan[1] != SMOKE_TILE_64x16) || h.plan[2] < 1)
+        h.plan[1] < SMOKE_TILE_32x32 || h.plan[1] > SMOKE_TILE_32x64 || h.plan[2] < 1)
         return fail(SMOKE_ERR_INVALID, "restore: inconsistent snapshot");
     smoke_status st = check_params(&h.prm);
     if (st != SMOKE_OK) return st;
@@ -1035,7 +1035,7 @@ smoke_status smoke_set_jacobi_plan(smoke_sim* s, const smoke_jacobi_plan* plan)
     if (plan->sweeps_per_launch < 1 || plan->sweeps_per_launch > 6)
         return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: sweeps_per_launch %d not in 1..6",
                     plan->sweeps_per_launch);
-    if (plan->tile != SMOKE_TILE_32x32 && plan->tile != SMOKE_TILE_64x16)
+    if (plan->tile < SMOKE_TILE_32x32 || plan->tile > SMOKE_TILE_32x64)
         return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: unknown tile %d", plan->tile);
     if (plan->zc < 1) return fail(SMOKE_ERR_INVALID, "set_jacobi_plan: zc %d < 1", plan->zc);
     std::lock_guard<std::mutex> lk(s->mu);
@@ -1072,8 +1072,16 @@ smoke_status smoke_autotune(smoke_sim* s, smoke_jacobi_plan* chosen, double* ms_
     }
     smk::JacobiPlan best = s->plan;
     double best_ms = -1.0;
-    const int tiles[2] = {smk::JacobiPlan::TILE_32x32, smk::JacobiPlan::TILE_64x16};
-    const int zcs[3] = {32, 64, 128};
+    const int tiles[4] = {smk::JacobiPlan::TILE_32x32, smk::JacobiPlan::TILE_64x16,
+                          smk::JacobiPlan::TILE_64x32, smk::JacobiPlan::TILE_32x64};
+    // z-chunks: fixed lengths, and the interior split into 1..4 equal chunks
+    // (one chunk per block gives the fewest blocks and no z halo; when the
+    // tiles of a plane about fill the GPU, that is fastest).
+    std::vector<int> zcs = {32, 64, 128};
+    for (int c = 1; c <= 4; ++c) {
+        const int zc = (s->g.nz - 2 + c - 1) / c;
+        if (zc >= 1 && std::find(zcs.begin(), zcs.end(), zc) == zcs.end()) zcs.push_back(zc);
+    }
     constexpr int kReps = 2;
     hipError_t e = hipSuccess;
     for (int t = 2; t <= 6 && e == hipSuccess; ++t)

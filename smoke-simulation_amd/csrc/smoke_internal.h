@@ -94,11 +94,17 @@ __host__ __device__ inline int fused_max_zc(long long plane)
 }
 
 struct JacobiPlan {
-    enum Tile { TILE_32x32 = 0, TILE_64x16 = 1 };
+    // Tiles 64x32 and 32x64 give each thread two rows of its column.
+    enum Tile { TILE_32x32 = 0, TILE_64x16 = 1, TILE_64x32 = 2, TILE_32x64 = 3 };
     int sweeps_per_launch = 4;
     int tile = TILE_32x32;
     int zc = 64;
 };
+inline int tile_width(int tile) { return (tile == JacobiPlan::TILE_32x32 || tile == JacobiPlan::TILE_32x64) ? 32 : 64; }
+inline int tile_height(int tile)
+{
+    return tile == JacobiPlan::TILE_64x16 ? 16 : (tile == JacobiPlan::TILE_32x64 ? 64 : 32);
+}
 
 // ------------------------------------------------------------- batching
 

@@ -372,31 +372,33 @@ static_assert(fused_slack_planes(2) <= kFusedSlackPlanes && fused_slack_planes(3
               "z-chunk slack of the fused sweeps");
 
 // T fused Jacobi sweeps (temporal blocking) for one tile (tile_x, tile_y) and
-// z-chunk tile_z of a TX x TY thread block.  The block computes the tile
+// z-chunk tile_z of a TX x TY tile, run by TX x (TY / R) threads: each thread
+// holds R adjacent tile rows of one column.  The block computes the tile
 // grown by T cells on each side in x and y and writes only the inner
 // (TX-2T) x (TY-2T) columns, whose values after T sweeps depend only on data
-// inside the grown tile.  Each thread holds one column and streams it in z:
-// level t (sweep t) at plane k-t is computed from level t-1 at planes
-// k-t-1, k-t, k-t+1 kept in registers and from the four x/y neighbours of
-// level t-1 at plane k-t (x from the adjacent lanes, y from LDS, double
-// buffered, one barrier per plane).  Every value is computed with the
-// single-sweep arithmetic, so results are bit-identical to T single sweeps.
-// Planes are prefetched into registers kPrefetch iterations ahead.  The block
-// reads planes [plo, phi) of P, d and flags through buffer descriptors over
-// exactly that span: planes outside it and lanes outside the grid (offsets
-// from 2^31) read 0, so the loads need no validity selects.  Requires
-// zc <= fused_max_zc(plane, T).  All threads of the block must call this with
-// the same tile (the barriers are block-wide).
+// inside the grown tile.  Each thread streams its cells in z: level t (sweep
+// t) at plane k-t is computed from level t-1 at planes k-t-1, k-t, k-t+1 kept
+// in registers and from the four x/y neighbours of level t-1 at plane k-t (x
+// from the adjacent lanes; y from the thread's own other rows, or from LDS
+// for the rows beyond its first and last, double buffered, one barrier per
+// plane).  Every value is computed with the single-sweep arithmetic, so
+// results are bit-identical to T single sweeps.  Planes are prefetched into
+// registers kPrefetch iterations ahead.  The block reads planes [plo, phi) of
+// P, d and flags through buffer descriptors over exactly that span: planes
+// outside it and lanes outside the grid (offsets from 2^31) read 0, so the
+// loads need no validity selects.  Requires zc <= fused_max_zc(plane, T).
+// All threads of the block must call this with the same tile (the barriers
+// are block-wide).
 //
 // Level t is valid only on tile rows [t, TY - t) (a row needs its two
 // neighbour rows of level t-1), and valid rows read only valid rows.  A wave
-// holds 64 / TX whole rows, so a wave none of whose rows is valid at level t
-// skips that level and every level above: it runs jacobi_fused_levels with NL
-// < T computed levels (a wave-uniform choice).  Every variant runs the same
-// plane loop with the same barriers, stores its levels 0..NL into LDS for the
-// rows above and below, and writes no output (written rows lie in waves with
-// NL = T).
-template <int TX, int TY, int T, bool DAMPED, int NL>
+// holds R * 64 / TX whole rows, so a wave none of whose rows is valid at
+// level t skips that level and every level above: it runs
+// jacobi_fused_levels with NL < T computed levels (a wave-uniform choice).
+// Every variant runs the same plane loop with the same barriers, stores its
+// levels 0..NL into LDS for the rows above and below, and writes no output
+// (written rows lie in waves with NL = T).
+template <int TX, int TY, int T, bool DAMPED, int R, int NL>
 __device__ __forceinline__ void jacobi_fused_levels(const Grid& g, const float* __restrict__ pin,
                                                     float* __restrict__ pout,
                                                     const float* __restrict__ d,
@@ -406,15 +408,16 @@ __device__ __forceinline__ void jacobi_fused_levels(const Grid& g, const float* 
                                                     FusedLds<TX, TY, T>& sh)
 {
     static_assert(NL >= 0 && NL <= T, "levels of a wave");
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    static_assert(R >= 1 && TY % R == 0, "whole tile rows per thread");
+    const int tx = threadIdx.x;
+    const int r0 = (int)threadIdx.y * R;  // first tile row of this thread
     constexpr int OX = TX - 2 * T, OY = TY - 2 * T;
     const int x = tile_x * OX + tx - T + 1;
-    const int y = tile_y * OY + ty - T + 1;
+    const int y0 = tile_y * OY + r0 - T + 1;
     const int z0 = tile_z * zc + 1;
     const int z1 = min(z0 + zc, g.nz - 1);
-    const bool in_grid = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
-    const bool writes = tx >= T && tx < TX - T && ty >= T && ty < TY - T && x <= g.nx - 2 &&
-                        y <= g.ny - 2;
+    const bool x_in = x >= 0 && x < g.nx;
+    const bool x_writes = tx >= T && tx < TX - T && x <= g.nx - 2;
     const int plo = max(z0 - T, 0), phi = min(z1 + T, g.nz);
     const uint32_t plane = (uint32_t)g.plane;
     const uint32_t span = (uint32_t)(phi - plo) * plane;  // elements
@@ -424,34 +427,48 @@ __device__ __forceinline__ void jacobi_fused_levels(const Grid& g, const float* 
     const __amdgpu_buffer_rsrc_t rd = buf_rsrc(d + first, span * 4u);
     const __amdgpu_buffer_rsrc_t rf = buf_rsrc(flags + first, span);
     const __amdgpu_buffer_rsrc_t ro = buf_rsrc(pout + first, span * 4u);
-    const uint32_t cell = in_grid ? (uint32_t)g.idx(x, y, 0) : 0u;
-    const uint32_t lane_b = in_grid ? cell * 4u : 0x80000000u;  // byte offset in plane plo
-    const uint32_t lane_e = in_grid ? cell : 0x80000000u;       // element offset
-    for (int f = tx + TX * ty; f < 256; f += TX * TY) sh.inv[f] = cell_inv((uint8_t)f);
+    uint32_t lane_b[R], lane_e[R];  // byte / element offsets in plane plo
+    bool writes[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int y = y0 + j, r = r0 + j;
+        const bool in_grid = x_in && y >= 0 && y < g.ny;
+        writes[j] = x_writes && r >= T && r < TY - T && y <= g.ny - 2;
+        const uint32_t cell = in_grid ? (uint32_t)g.idx(x, y, 0) : 0u;
+        lane_b[j] = in_grid ? cell * 4u : 0x80000000u;
+        lane_e[j] = in_grid ? cell : 0x80000000u;
+    }
+    for (int f = tx + TX * (int)threadIdx.y; f < 256; f += TX * (TY / R)) sh.inv[f] = cell_inv((uint8_t)f);
     __syncthreads();
 
-    float w[T][3];  // level t at planes (newest-2, newest-1, newest)
+    float w[R][T][3];  // level t at planes (newest-2, newest-1, newest)
     // Inverse and negated scaled right-hand side 0 - d*inv of planes k, k-1,
     // ..., k-T.  On cells that are not active inv = 0 and d = 0, so nq = +0
     // and the plain update fma(sum, 0, +0) is +0 with no select.
-    float iq[T + 1], dq[T + 1];
+    float iq[R][T + 1], dq[R][T + 1];
 #pragma unroll
-    for (int t = 0; t < T; ++t) w[t][0] = w[t][1] = w[t][2] = 0.0f;
+    for (int j = 0; j < R; ++j) {
 #pragma unroll
-    for (int t = 0; t <= T; ++t) iq[t] = dq[t] = 0.0f;
+        for (int t = 0; t < T; ++t) w[j][t][0] = w[j][t][1] = w[j][t][2] = 0.0f;
+#pragma unroll
+        for (int t = 0; t <= T; ++t) iq[j][t] = dq[j][t] = 0.0f;
+    }
 
-    // Prefetch ring: plane k + j in slot j.  Planes below plo wrap to offsets
+    // Prefetch ring: plane k + s in slot s.  Planes below plo wrap to offsets
     // near 2^32 and planes at or past phi lie past the span: both read 0.
-    float np[kPrefetch], nd[kPrefetch];
-    uint8_t nf[kPrefetch];
-    auto load = [&](int kn, float& p, float& dv, uint8_t& f) {
+    float np[R][kPrefetch], nd[R][kPrefetch];
+    uint8_t nf[R][kPrefetch];
+    auto load = [&](int kn, int s) {
         const uint32_t dk = (uint32_t)(kn - plo) * plane;
-        p = buf_f32(rp, lane_b + dk * 4u);
-        dv = buf_f32(rd, lane_b + dk * 4u);
-        f = buf_u8(rf, lane_e + dk);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            np[j][s] = buf_f32(rp, lane_b[j] + dk * 4u);
+            nd[j][s] = buf_f32(rd, lane_b[j] + dk * 4u);
+            nf[j][s] = buf_u8(rf, lane_e[j] + dk);
+        }
     };
 #pragma unroll
-    for (int j = 0; j < kPrefetch; ++j) load(z0 - T + j, np[j], nd[j], nf[j]);
+    for (int s = 0; s < kPrefetch; ++s) load(z0 - T + s, s);
     // The trip count is rounded up to a multiple of the unroll factor; the
     // extra iterations load nothing in range and write nothing (zo >= z1).
     constexpr int U = z_unroll(T);
@@ -460,62 +477,88 @@ __device__ __forceinline__ void jacobi_fused_levels(const Grid& g, const float* 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int k = kb + u;
-        const float p0 = np[0], dd = nd[0];
-        const uint8_t fl = nf[0];
+        float p0[R], dd[R];
+        uint8_t fl[R];
 #pragma unroll
-        for (int j = 0; j + 1 < kPrefetch; ++j) {
-            np[j] = np[j + 1];
-            nd[j] = nd[j + 1];
-            nf[j] = nf[j + 1];
-        }
-        load(k + kPrefetch, np[kPrefetch - 1], nd[kPrefetch - 1], nf[kPrefetch - 1]);
+        for (int j = 0; j < R; ++j) {
+            p0[j] = np[j][0];
+            dd[j] = nd[j][0];
+            fl[j] = nf[j][0];
 #pragma unroll
-        for (int t = T; t > 0; --t) {
-            iq[t] = iq[t - 1];
-            dq[t] = dq[t - 1];
+            for (int s = 0; s + 1 < kPrefetch; ++s) {
+                np[j][s] = np[j][s + 1];
+                nd[j][s] = nd[j][s + 1];
+                nf[j][s] = nf[j][s + 1];
+            }
         }
-        iq[0] = sh.inv[fl];
-        dq[0] = neg_scaled_rhs(dd, iq[0]);
-        w[0][0] = w[0][1];
-        w[0][1] = w[0][2];
-        w[0][2] = p0;
+        load(k + kPrefetch, kPrefetch - 1);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+#pragma unroll
+            for (int t = T; t > 0; --t) {
+                iq[j][t] = iq[j][t - 1];
+                dq[j][t] = dq[j][t - 1];
+            }
+            iq[j][0] = sh.inv[fl[j]];
+            dq[j][0] = neg_scaled_rhs(dd[j], iq[j][0]);
+            w[j][0][0] = w[j][0][1];
+            w[j][0][1] = w[j][0][2];
+            w[j][0][2] = p0[j];
+        }
         const int cur = k & 1, prev = cur ^ 1;
-        float newv[T + 1];
-        newv[0] = p0;
+        float newv[R][T + 1];
+#pragma unroll
+        for (int j = 0; j < R; ++j) newv[j][0] = p0[j];
 #pragma unroll
         for (int t = 1; t <= NL; ++t) {
-            // x neighbours of level t-1 at plane k-t from the adjacent lanes
-            // (w[t-1][1] holds that plane for every lane), y neighbours from LDS.
-            const float sum = sum6(lane_prev(w[t - 1][1]), lane_next(w[t - 1][1]),
-                                   sh.v[t - 1][prev][ty][tx + 1], sh.v[t - 1][prev][ty + 2][tx + 1],
-                                   w[t - 1][0], w[t - 1][2]);
-            const float G = __builtin_fmaf(sum, iq[t], dq[t]);
-            float v;
-            if (DAMPED) {
-                const float pc = w[t - 1][1];
-                v = pc + omega * (G - pc);
-                v = (iq[t] > 0.0f) ? v : 0.0f;
-            } else {
-                v = G;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                // x neighbours of level t-1 at plane k-t from the adjacent
+                // lanes (w[j][t-1][1] holds that plane for every lane); y
+                // neighbours from this thread's rows j-1 / j+1, or from LDS
+                // (tile row r lives in LDS row r + 1).
+                const float ym = (j > 0) ? w[j > 0 ? j - 1 : 0][t - 1][1]
+                                         : sh.v[t - 1][prev][r0][tx + 1];
+                const float yp = (j + 1 < R) ? w[j + 1 < R ? j + 1 : 0][t - 1][1]
+                                             : sh.v[t - 1][prev][r0 + R + 1][tx + 1];
+                const float sum = sum6(lane_prev(w[j][t - 1][1]), lane_next(w[j][t - 1][1]), ym,
+                                       yp, w[j][t - 1][0], w[j][t - 1][2]);
+                const float G = __builtin_fmaf(sum, iq[j][t], dq[j][t]);
+                float v;
+                if (DAMPED) {
+                    const float pc = w[j][t - 1][1];
+                    v = pc + omega * (G - pc);
+                    v = (iq[j][t] > 0.0f) ? v : 0.0f;
+                } else {
+                    v = G;
+                }
+                newv[j][t] = v;
             }
-            newv[t] = v;
             if (t < NL) {
-                w[t][0] = w[t][1];
-                w[t][1] = w[t][2];
-                w[t][2] = v;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    w[j][t][0] = w[j][t][1];
+                    w[j][t][1] = w[j][t][2];
+                    w[j][t][2] = newv[j][t];
+                }
             }
         }
         const int zo = k - T;
-        if (NL == T && writes && zo >= z0 && zo < z1)
-            buf_st_f32(ro, lane_b + (uint32_t)(zo - plo) * plane * 4u, newv[NL]);
+        if (NL == T && zo >= z0 && zo < z1) {
 #pragma unroll
-        for (int t = 0; t < T && t <= NL; ++t) sh.v[t][cur][ty + 1][tx + 1] = newv[t];
+            for (int j = 0; j < R; ++j)
+                if (writes[j]) buf_st_f32(ro, lane_b[j] + (uint32_t)(zo - plo) * plane * 4u, newv[j][NL]);
+        }
+#pragma unroll
+        for (int t = 0; t < T && t <= NL; ++t)
+#pragma unroll
+            for (int j = 0; j < R; ++j) sh.v[t][cur][r0 + j + 1][tx + 1] = newv[j][t];
         __syncthreads();
     }
 }
 
 // The wave's level count NL (see jacobi_fused_levels) as a template argument.
-template <int TX, int TY, int T, bool DAMPED, int K>
+template <int TX, int TY, int T, bool DAMPED, int R, int K>
 __device__ __forceinline__ void jacobi_fused_dispatch(int nl, const Grid& g,
                                                       const float* __restrict__ pin,
                                                       float* __restrict__ pout,
@@ -526,20 +569,21 @@ __device__ __forceinline__ void jacobi_fused_dispatch(int nl, const Grid& g,
                                                       FusedLds<TX, TY, T>& sh)
 {
     if constexpr (K == 0) {
-        jacobi_fused_levels<TX, TY, T, DAMPED, 0>(g, pin, pout, d, flags, zc, zero_in, omega,
-                                                  tile_x, tile_y, tile_z, sh);
+        jacobi_fused_levels<TX, TY, T, DAMPED, R, 0>(g, pin, pout, d, flags, zc, zero_in, omega,
+                                                     tile_x, tile_y, tile_z, sh);
     } else {
         if (nl == K)
-            jacobi_fused_levels<TX, TY, T, DAMPED, K>(g, pin, pout, d, flags, zc, zero_in, omega,
-                                                      tile_x, tile_y, tile_z, sh);
+            jacobi_fused_levels<TX, TY, T, DAMPED, R, K>(g, pin, pout, d, flags, zc, zero_in,
+                                                         omega, tile_x, tile_y, tile_z, sh);
         else
-            jacobi_fused_dispatch<TX, TY, T, DAMPED, K - 1>(nl, g, pin, pout, d, flags, zc,
-                                                            zero_in, omega, tile_x, tile_y,
-                                                            tile_z, sh);
+            jacobi_fused_dispatch<TX, TY, T, DAMPED, R, K - 1>(nl, g, pin, pout, d, flags, zc,
+                                                               zero_in, omega, tile_x, tile_y,
+                                                               tile_z, sh);
     }
 }
 
-template <int TX, int TY, int T, bool DAMPED>
+// Block of TX x (TY / R) threads; R tile rows per thread.
+template <int TX, int TY, int T, bool DAMPED, int R = 1>
 __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __restrict__ pin,
                                                   float* __restrict__ pout,
                                                   const float* __restrict__ d,
@@ -548,13 +592,14 @@ __device__ __forceinline__ void jacobi_fused_tile(const Grid& g, const float* __
                                                   int tile_z, FusedLds<TX, TY, T>& sh)
 {
     static_assert(64 % TX == 0, "a wave holds whole tile rows");
-    constexpr int RPW = 64 / TX;  // rows per wave
-    // First row of this thread's wave (uniform across the wave) and the
+    constexpr int TPW = 64 / TX;  // thread rows per wave
+    constexpr int RPW = TPW * R;  // tile rows per wave
+    // First tile row of this thread's wave (uniform across the wave) and the
     // highest level with a valid row in the wave: t <= last row, t <= TY-1-first row.
-    const int r0 = __builtin_amdgcn_readfirstlane((int)threadIdx.y / RPW * RPW);
+    const int r0 = __builtin_amdgcn_readfirstlane((int)threadIdx.y / TPW * TPW) * R;
     const int nl = min(T, min(r0 + RPW - 1, TY - 1 - r0));
-    jacobi_fused_dispatch<TX, TY, T, DAMPED, T>(nl, g, pin, pout, d, flags, zc, zero_in, omega,
-                                                tile_x, tile_y, tile_z, sh);
+    jacobi_fused_dispatch<TX, TY, T, DAMPED, R, T>(nl, g, pin, pout, d, flags, zc, zero_in, omega,
+                                                   tile_x, tile_y, tile_z, sh);
 }
 
 // u -= grad P on every face whose two cells are fluid.  Faces that touch no

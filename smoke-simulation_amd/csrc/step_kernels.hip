@@ -330,17 +330,19 @@ __device__ __forceinline__ int xcd_tile_order()
     return (b < full) ? (b & 7) * (full >> 3) + (b >> 3) : b;
 }
 
-// T fused Jacobi sweeps, one tile per block (dev::jacobi_fused_tile).
-template <int TX, int TY, int T, bool DAMPED>
-__global__ __launch_bounds__(TX * TY, 8) void k_jacobi_fused(
+// T fused Jacobi sweeps, one TX x TY tile per block of TX x (TY / R) threads
+// (dev::jacobi_fused_tile).  One row per thread: at most 64 VGPRs, two
+// 1024-thread blocks per CU; two rows per thread: at most 128 VGPRs.
+template <int TX, int TY, int T, bool DAMPED, int R>
+__global__ __launch_bounds__(TX * TY / R, 8 / R) void k_jacobi_fused(
     Grid g, const float* __restrict__ pin, float* __restrict__ pout, const float* __restrict__ d,
     const uint8_t* __restrict__ flags, int zc, int zero_in, float omega)
 {
     __shared__ dev::FusedLds<TX, TY, T> sh;
     const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
     const int L = xcd_tile_order();
-    dev::jacobi_fused_tile<TX, TY, T, DAMPED>(g, pin, pout, d, flags, zc, zero_in, omega, L % nbx,
-                                              (L / nbx) % nby, L / (nbx * nby), sh);
+    dev::jacobi_fused_tile<TX, TY, T, DAMPED, R>(g, pin, pout, d, flags, zc, zero_in, omega,
+                                                 L % nbx, (L / nbx) % nby, L / (nbx * nby), sh);
 }
 
 // One red-black SOR half-sweep of the pressure form, in place.
@@ -640,7 +642,7 @@ hipError_t jacobi_sweep(const Grid& g, const float* pin, float* pout, const floa
     return hipGetLastError();
 }
 
-template <int TX, int TY, int T>
+template <int TX, int TY, int R, int T>
 static hipError_t launch_fused(const Grid& g, const float* pin, float* pout, const float* d,
                                const uint8_t* flags, int zc, int zero_in, float omega, bool damped,
                                hipStream_t st)
@@ -649,26 +651,27 @@ static hipError_t launch_fused(const Grid& g, const float* pin, float* pout, con
     const int m[3] = {g.nx - 2, g.ny - 2, g.nz - 2};
     dim3 grid((unsigned)((m[0] + OX - 1) / OX), (unsigned)((m[1] + OY - 1) / OY),
               (unsigned)((m[2] + zc - 1) / zc));
+    const dim3 block(TX, TY / R);
     if (damped)
-        hipLaunchKernelGGL((k_jacobi_fused<TX, TY, T, true>), grid, dim3(TX, TY), 0, st, g, pin, pout,
+        hipLaunchKernelGGL((k_jacobi_fused<TX, TY, T, true, R>), grid, block, 0, st, g, pin, pout,
                            d, flags, zc, zero_in, omega);
     else
-        hipLaunchKernelGGL((k_jacobi_fused<TX, TY, T, false>), grid, dim3(TX, TY), 0, st, g, pin,
+        hipLaunchKernelGGL((k_jacobi_fused<TX, TY, T, false, R>), grid, block, 0, st, g, pin,
                            pout, d, flags, zc, zero_in, omega);
     return hipGetLastError();
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int R>
 static hipError_t launch_fused_t(const Grid& g, const float* pin, float* pout, const float* d,
                                  const uint8_t* flags, int t, int zc, int zero_in, float omega,
                                  bool damped, hipStream_t st)
 {
     switch (t) {
-    case 2: return launch_fused<TX, TY, 2>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
-    case 3: return launch_fused<TX, TY, 3>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
-    case 4: return launch_fused<TX, TY, 4>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
-    case 5: return launch_fused<TX, TY, 5>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
-    case 6: return launch_fused<TX, TY, 6>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 2: return launch_fused<TX, TY, R, 2>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 3: return launch_fused<TX, TY, R, 3>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 4: return launch_fused<TX, TY, R, 4>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 5: return launch_fused<TX, TY, R, 5>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
+    case 6: return launch_fused<TX, TY, R, 6>(g, pin, pout, d, flags, zc, zero_in, omega, damped, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -687,9 +690,13 @@ hipError_t jacobi_fused(const Grid& g, const float* pin, float* pout, const floa
     const bool damped = !plain;
     switch (plan.tile) {
     case JacobiPlan::TILE_32x32:
-        return launch_fused_t<32, 32>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
+        return launch_fused_t<32, 32, 1>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
     case JacobiPlan::TILE_64x16:
-        return launch_fused_t<64, 16>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
+        return launch_fused_t<64, 16, 1>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
+    case JacobiPlan::TILE_64x32:
+        return launch_fused_t<64, 32, 2>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
+    case JacobiPlan::TILE_32x64:
+        return launch_fused_t<32, 64, 2>(g, pin, pout, d, flags, sweeps, zc, zero_in, omega, damped, st);
     default: return hipErrorInvalidValue;
     }
 }

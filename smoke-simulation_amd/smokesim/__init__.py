@@ -68,7 +68,7 @@ class JacobiPlan(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-TILE_32x32, TILE_64x16 = 0, 1
+TILE_32x32, TILE_64x16, TILE_64x32, TILE_32x64 = 0, 1, 2, 3
 
 
 class StepNorms(ctypes.Structure):

@@ -43,7 +43,7 @@ def case_plans():
     sc = scenes.test_scene((19, 14, 12), seed=4, projection=2, iterations=7)
     o = oracle_for(sc, 32)
     o.step(sc["dt"], 1)
-    for t, tile, zc in [(3, 1, 7), (6, 0, 4), (2, 1, 64)]:
+    for t, tile, zc in [(3, 1, 7), (6, 0, 4), (2, 1, 64), (6, 2, 5), (4, 3, 9), (5, 2, 64)]:
         g = gpu_for(sc)
         g.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
         g.step(sc["dt"], readback=False)
@@ -59,19 +59,22 @@ def case_batch():
                                n_obstacles=k % 2)
         sc["dt"] = 0.03 + 0.01 * k
         scs.append(sc)
-    sims = [gpu_for(sc) for sc in scs]
-    with smokesim.Batch(sims, [sc["dt"] for sc in scs]) as b:
-        b.step(use_graph=False)
-        for _ in range(2):
-            b.step(use_graph=True)
-        b.synchronize()
-        states = [s.state() for s in sims]
-    for sc, st in zip(scs, states):
-        o = oracle_for(sc, 32)
-        o.step(sc["dt"], 3)
-        assert_bitwise_equal(st, o.state(), f"batch {sc['name']}")
-    for s in sims:
-        s.close()
+    for plan in (None, (5, 2, 4), (3, 3, 64)):
+        sims = [gpu_for(sc) for sc in scs]
+        with smokesim.Batch(sims, [sc["dt"] for sc in scs]) as b:
+            if plan:
+                b.set_jacobi_plan(sweeps_per_launch=plan[0], tile=plan[1], zc=plan[2])
+            b.step(use_graph=False)
+            for _ in range(2):
+                b.step(use_graph=True)
+            b.synchronize()
+            states = [s.state() for s in sims]
+        for sc, st in zip(scs, states):
+            o = oracle_for(sc, 32)
+            o.step(sc["dt"], 3)
+            assert_bitwise_equal(st, o.state(), f"batch {sc['name']} plan {plan}")
+        for s in sims:
+            s.close()
 
 
 def _req(dims, seed, dt, its=6):

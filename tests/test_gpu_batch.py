@@ -57,7 +57,8 @@ def test_batch_equals_alone_and_oracle(use_graph, omega):
         assert_bitwise_equal(st, o.state(), f"{sc['name']} batch vs oracle")
 
 
-@pytest.mark.parametrize("plan", [(2, 0, 32), (3, 1, 7), (5, 0, 128), (1, 0, 64), (6, 1, 1)])
+@pytest.mark.parametrize("plan", [(2, 0, 32), (3, 1, 7), (5, 0, 128), (1, 0, 64), (6, 1, 1),
+                                  (5, 2, 64), (4, 3, 9), (6, 2, 256), (2, 2, 1)])
 def test_batch_plans(plan):
     t, tile, zc = plan
     scs = members(iterations=13)

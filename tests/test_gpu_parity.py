@@ -157,7 +157,8 @@ def test_footprint_matches_allocation_budget():
         pass
 
 
-PLANS = [(t, tile, zc) for t in (1, 2, 3, 4, 5, 6) for tile in (smokesim.TILE_32x32, smokesim.TILE_64x16)
+PLANS = [(t, tile, zc) for t in (1, 2, 3, 4, 5, 6)
+         for tile in (smokesim.TILE_32x32, smokesim.TILE_64x16, smokesim.TILE_64x32, smokesim.TILE_32x64)
          for zc in (1, 7, 128)]
 
 
