@@ -548,7 +548,11 @@ def run_batch(args, scs):
     k = scs[0]["params"]["iterations"]
     log(f"batch: {len(scs)} simulations, {cells} cells, Jacobi x{k}")
     tuned = None
-    if not args.no_autotune:
+    if args.plan:
+        t, tile, zc = (int(v) for v in args.plan.split(","))
+        b.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
+        log(f"batch plan: {b.get_jacobi_plan()}")
+    elif not args.no_autotune:
         plan, ms = b.autotune()
         tuned = {"plan": plan, "ms_per_sweep": round(ms, 5)}
         log(f"batch autotune: {plan}")
