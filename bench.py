@@ -214,7 +214,7 @@ def latency(ms):
 
 # The mixed-precision form: field storage (density and velocity in FP16),
 # projection (FP32 refinement around FP16 inner sweeps) and its label.
-MIXED = dict(projection=3, inner_sweeps=20)
+MIXED = dict(projection=3, inner_sweeps=50)
 MIXED_STORAGE = 3
 MIXED_DTYPE = "f16 density and velocity, f16 inner sweeps, f32 math"
 
@@ -321,10 +321,11 @@ def time_readback(sim, dt, n):
     return out
 
 
-def time_form(args, sc, storage, extra=None, plan=None, readback=False):
+def time_form(args, sc, storage, extra=None, plan=None, readback=False, args_plan=True):
     """Creates the simulation of `sc` with the given field storage (and extra
-    parameters), applies the Jacobi plan (`plan`, --plan, or the autotuner's
-    for the Jacobi projection), runs the warm-up steps, then times
+    parameters), applies the Jacobi plan (`plan`, --plan unless args_plan is
+    False, or the autotuner's for the Jacobi and refined projections), runs
+    the warm-up steps, then times
     args.steps full steps with HIP events per stage and around the sweep
     loop.  Returns the measurements."""
     import numpy as np
@@ -338,11 +339,11 @@ def time_form(args, sc, storage, extra=None, plan=None, readback=False):
     tuned = None
     if plan is not None:
         sim.set_jacobi_plan(**plan)
-    elif args.plan:
+    elif args.plan and args_plan:
         t, tile, zc = (int(v) for v in args.plan.split(","))
         sim.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
         log(f"plan: {sim.get_jacobi_plan()}")
-    elif sim.get_params()["projection"] == 2 and not args.no_autotune:
+    elif sim.get_params()["projection"] in (2, 3) and not args.no_autotune:
         plan_, ms = sim.autotune()
         tuned = {"plan": plan_, "ms_per_sweep": round(ms, 5)}
         log(f"autotune: {plan_} at {ms * 1e3:.2f} us/sweep")
@@ -408,14 +409,15 @@ def run_single(args, sc):
         avg_ms = r["sweep_ms"] / r["sweep_launches"]
         sweeps_per_launch = None
         if p["projection"] == 3:
-            # The loop also holds the norm and residual passes, so the
-            # loop's compulsory bytes are averaged over its launches.
+            # The loop's time (norm pass, FP16 sweep launches, residual
+            # passes) is divided by its FP16 sweep launches, so the loop's
+            # compulsory bytes are averaged over the same launches.
             refinements = -(-p["iterations"] // p["inner_sweeps"])
             t = r["plan"]["sweeps_per_launch"]
             t = t if 2 <= t <= 6 and t % 2 == 0 else 4
             per_ref = -(-p["inner_sweeps"] // t)
             loop = BYTES_REF_NORM0 + refinements * (per_ref * BYTES_REF_SWEEP + BYTES_REF_RESID)
-            launches = 1 + refinements * (per_ref + 1)
+            launches = refinements * per_ref
             comp = loop * interior / launches
             unfused = (BYTES_REF_NORM0 + refinements * (p["inner_sweeps"] * BYTES_REF_SWEEP
                                                         + BYTES_REF_RESID)) * interior / launches
@@ -439,13 +441,16 @@ def run_single(args, sc):
             "alg_bytes_per_launch": int(comp),
             "alg_bytes_model": "compulsory: 13 B x interior cells per fused launch (P in, d, "
                                "flags, P out once, whatever the sweeps fused)"
-                               if kernel == "k_jacobi_fused" else "compulsory per launch",
+                               if kernel == "k_jacobi_fused" else
+                               ("compulsory bytes of the refined loop (norm pass, FP16 sweeps, "
+                                "residual passes) per FP16 sweep launch" if kernel == "k_ref_sweeps"
+                                else "compulsory per launch"),
             "sweeps_per_launch": sweeps_per_launch,
             "effective": {"bytes_per_launch": int(unfused),
                           "frac": round(unfused / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "model": "unfused: 13 B x interior cells x sweeps per launch"},
             "launches_timed": r["sweep_launches"],
-            "jacobi_plan": r["plan"] if p["projection"] == 2 else None,
+            "jacobi_plan": r["plan"] if p["projection"] in (2, 3) else None,
             "autotune": r["autotune"],
         }
         traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
@@ -506,7 +511,7 @@ def run_single(args, sc):
         # The mixed-precision form on the same inputs, same sweep count.
         log("mixed-precision form ...")
         m = time_form(args, sc, MIXED_STORAGE, dict(MIXED, iterations=fp32_iterations),
-                      plan=dict(sweeps_per_launch=4, tile=0, zc=64))
+                      args_plan=False)
         mms = 1000.0 * m["wall"] / args.steps
         out["mixed"] = {
             "dtype": MIXED_DTYPE, "ms_per_step": round(mms, 4),
@@ -517,6 +522,8 @@ def run_single(args, sc):
                                   for k, v in m["timing"]["stage_ms"].items()},
             "solve": ({"relative_residual": m["solve"][0], "refinements": m["solve"][1]}
                       if m["solve"] else None),
+            "params": {k: m["params"][k] for k in ("projection", "iterations", "inner_sweeps")},
+            "plan": m["plan"], "autotune": m["autotune"],
             "step_norms": m["norms"],
         }
     if args.error_steps:

@@ -204,7 +204,9 @@ smoke_status smoke_get_jacobi_plan(const smoke_sim* sim, smoke_jacobi_plan* out)
  * iteration count, or 100 if that is 0) on this simulation's buffers for
  * every candidate plan -- 2..6 sweeps per launch x the four tiles x z-chunks
  * of 32, 64 and 128 planes and of the interior's nz-2 planes split into 1..4
- * equal chunks -- and keeps the fastest.  Results are identical for
+ * equal chunks -- and keeps the fastest.  With the REFINED projection it
+ * times the refined solve instead (FP16 sweep launches of 2, 4 or 6 sweeps,
+ * one-row 32x32 or two-row 64x32 tiles, the same z-chunks).  Results are identical for
  * every plan, so tuning never changes answers.  Resets the pressure kept for
  * warm starts to zero.  `chosen` (may be NULL) receives the plan and
  * `ms_per_sweep` (may be NULL) its measured time per sweep. */

@@ -294,34 +294,246 @@ struct HalfLds {
 };
 
 // T fused FP16 Jacobi sweeps on the correction for one tile, temporal blocking
-// as in the FP32 kernel (step_device.h: jacobi_fused_tile), with each thread
-// owning a pair of adjacent columns (x0, x0+1), x0 even.  The grown tile is
-// 2*TXP cells wide; the inner 2*TXP - 2T columns and TY - 2T rows are written
-// (T even, so the inner region is whole pairs).  Pairs cover x in [0, px): the
-// cells beyond the interior have inv = 0 and r = 0, so they stay e = 0.  The
-// diagonal inverse of a cell is the FP16 rounding of cell_inv(flags byte),
-// from a 256-entry LDS table: the lane loads its pair's two flags bytes (one
-// byte per cell instead of a two-byte FP16 stream).  Flags beyond the grid
-// (padding columns, row and plane) are 0 from creation on (only k_flags writes
-// flags, over the grid), so they give inv = 0 there.  FIRST:
-// e starts at 0 and the scaled right-hand side inv*r*s is formed from the FP32
-// r and written for the later launches of the refinement.
-template <int TXP, int TY, int T, bool FIRST>
-__global__ __launch_bounds__(TXP * TY, 8) void k_ref_sweeps(Grid g, const uint32_t* __restrict__ ein,
-                                                         uint32_t* __restrict__ eout,
-                                                         const float* __restrict__ q,
-                                                         uint32_t* __restrict__ qi,
-                                                         const uint16_t* __restrict__ fl2,
-                                                         const RefineCtl* __restrict__ ctl, int zc)
+// as in the FP32 kernel (step_device.h: jacobi_fused_levels), with each thread
+// owning a pair of adjacent columns (x0, x0+1), x0 even, in R adjacent tile
+// rows.  The grown tile is 2*TXP cells wide and TY rows high; the inner
+// 2*TXP - 2T columns and TY - 2T rows are written (T even, so the inner
+// region is whole pairs).  Pairs cover x in [0, px): the cells beyond the
+// interior have inv = 0 and r = 0, so they stay e = 0.  The diagonal inverse
+// of a cell is the FP16 rounding of cell_inv(flags byte), from a 256-entry
+// LDS table: the lane loads its pair's two flags bytes (one byte per cell
+// instead of a two-byte FP16 stream).  Flags beyond the grid (padding
+// columns, row and plane) are 0 from creation on (only k_flags writes flags,
+// over the grid), so they give inv = 0 there.  FIRST: e starts at 0 and the
+// scaled right-hand side inv*r*s is formed from the FP32 r and written for
+// the later launches of the refinement.
+//
+// Loads and stores go through buffer descriptors over the block's planes
+// [plo, phi): a plane outside the span or a lane outside the grid (offsets
+// from 2^31) reads 0, as the cells there must.  A wave none of whose rows is
+// valid at level t skips that level and the ones above (NL < T), as in the
+// FP32 kernel.
+template <int TXP, int TY, int T, bool FIRST, int R, int NL>
+__device__ __forceinline__ void ref_sweep_levels(const Grid& g, const uint32_t* __restrict__ ein,
+                                                 uint32_t* __restrict__ eout,
+                                                 const float* __restrict__ q,
+                                                 uint32_t* __restrict__ qi,
+                                                 const uint16_t* __restrict__ fl2, float s, int zc,
+                                                 int tile_x, int tile_y, int tile_z,
+                                                 HalfLds<TXP, TY, T>& sh)
+{
+    static_assert(NL >= 0 && NL <= T, "levels of a wave");
+    const int tx = threadIdx.x;
+    const int r0 = (int)threadIdx.y * R;  // first tile row of this thread
+    constexpr int OXC = 2 * TXP - 2 * T, OY = TY - 2 * T;
+    const int x0 = tile_x * OXC + 2 * tx - T;
+    const int z0 = tile_z * zc + 1;
+    const int z1 = min(z0 + zc, g.nz - 1);
+    const int plo = max(z0 - T, 0), phi = min(z1 + T, g.nz + 1);
+    const uint32_t wplane = (uint32_t)g.plane >> 1;       // pairs per plane
+    const uint32_t span = (uint32_t)(phi - plo) * wplane;  // pairs
+    const size_t firstw = (size_t)plo * wplane;
+    const __amdgpu_buffer_rsrc_t rf = dev::buf_rsrc(fl2 + firstw, span * 2u);
+    const __amdgpu_buffer_rsrc_t re = dev::buf_rsrc(ein + firstw, FIRST ? 0u : span * 4u);
+    const __amdgpu_buffer_rsrc_t rq = dev::buf_rsrc(qi + firstw, span * 4u);
+    const __amdgpu_buffer_rsrc_t rr = dev::buf_rsrc(q + 2 * firstw, FIRST ? span * 8u : 0u);
+    const __amdgpu_buffer_rsrc_t ro = dev::buf_rsrc(eout + firstw, span * 4u);
+    uint32_t o4[R];  // byte offset of the lane's pair word in plane plo (0x80000000: outside)
+    bool writes[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int r = r0 + j;
+        const int y = tile_y * OY + r - T + 1;
+        const bool in_grid = x0 >= 0 && x0 < g.px && y >= 0 && y <= g.ny;
+        writes[j] = tx >= T / 2 && tx < TXP - T / 2 && r >= T && r < TY - T && x0 >= 0 &&
+                    x0 < g.px && y >= 1 && y <= g.ny - 2;
+        o4[j] = in_grid ? (uint32_t)(g.idx(x0, y, 0) >> 1) * 4u : 0x80000000u;
+    }
+
+    const h2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
+    h2 w[R][T][3];
+    h2 iq[R][T + 1], qq[R][T + 1];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) w[j][t][0] = w[j][t][1] = w[j][t][2] = zero2;
+#pragma unroll
+        for (int t = 0; t <= T; ++t) iq[j][t] = qq[j][t] = zero2;
+    }
+
+    // Prefetch ring (plane k + i in slot i), dev::kPrefetch planes deep.  The
+    // ring holds loaded values only (FIRST: the raw FP32 r); anything computed
+    // from them is computed when the plane is consumed, so no wait is forced
+    // at the load.  Offsets: 4 bytes per pair word, 2 per flags pair, 8 per
+    // FP32 pair (a lane outside the grid stays past every span).
+    constexpr int D = dev::kPrefetch;
+    uint32_t ne[R][D], nq[R][D];
+    uint16_t nfl[R][D];  // the pair's two flags bytes
+    float nr0[R][D], nr1[R][D];
+    auto load = [&](int kn, int i) {
+        const uint32_t dk = (uint32_t)(kn - plo) * wplane;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const uint32_t o = o4[j] + dk * 4u;
+            nfl[j][i] = __builtin_amdgcn_raw_buffer_load_b16(rf, (int)(o >> 1), 0, 0);
+            if (FIRST) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, (int)(o << 1), 0, 0);
+                // Elements through scalars: a bit cast of v[1] itself reads
+                // element 0 with this compiler.
+                const uint32_t v0 = v[0], v1 = v[1];
+                nr0[j][i] = __builtin_bit_cast(float, v0);
+                nr1[j][i] = __builtin_bit_cast(float, v1);
+            } else {
+                ne[j][i] = __builtin_amdgcn_raw_buffer_load_b32(re, (int)o, 0, 0);
+                nq[j][i] = __builtin_amdgcn_raw_buffer_load_b32(rq, (int)o, 0, 0);
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < D; ++i) load(z0 - T + i, i);
+    // Unrolled by lcm(3, T+1) so the register windows rotate without copies;
+    // the padded iterations load and write nothing.
+    constexpr int U = dev::z_unroll(T);
+    const int kend = z0 - T + ((z1 - z0 + 2 * T + U - 1) / U) * U;
+    for (int kb = z0 - T; kb < kend; kb += U)
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int k = kb + u;
+        h2 qv[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const uint16_t fp = nfl[j][0];  // 0 outside: inv = 0
+            const h2 ii = {__builtin_bit_cast(_Float16, sh.itab[fp & 0xff]),
+                           __builtin_bit_cast(_Float16, sh.itab[fp >> 8])};
+            const h2 p0 = FIRST ? zero2 : __builtin_bit_cast(h2, ne[j][0]);
+            if (FIRST) {
+                qv[j].x = dev::to_f16(nr0[j][0] * s * (float)ii.x);
+                qv[j].y = dev::to_f16(nr1[j][0] * s * (float)ii.y);
+            } else {
+                qv[j] = __builtin_bit_cast(h2, nq[j][0]);
+            }
+#pragma unroll
+            for (int i = 0; i + 1 < D; ++i) {
+                ne[j][i] = ne[j][i + 1];
+                nfl[j][i] = nfl[j][i + 1];
+                nq[j][i] = nq[j][i + 1];
+                nr0[j][i] = nr0[j][i + 1];
+                nr1[j][i] = nr1[j][i + 1];
+            }
+#pragma unroll
+            for (int t = T; t > 0; --t) {
+                iq[j][t] = iq[j][t - 1];
+                qq[j][t] = qq[j][t - 1];
+            }
+            iq[j][0] = ii;
+            qq[j][0] = qv[j];
+            w[j][0][0] = w[j][0][1];
+            w[j][0][1] = w[j][0][2];
+            w[j][0][2] = p0;
+        }
+        load(k + D, D - 1);
+        const int cur = k & 1, prev = cur ^ 1;
+        h2 newv[R][T + 1];
+#pragma unroll
+        for (int j = 0; j < R; ++j) newv[j][0] = w[j][0][2];
+#pragma unroll
+        for (int t = 1; t <= NL; ++t) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                // x neighbours from the adjacent lanes (DPP); y neighbours from
+                // this thread's rows j-1 / j+1, or from LDS (tile row r lives
+                // in LDS row r + 1).
+                const h2 C = w[j][t - 1][1];
+                const uint32_t cw = __builtin_bit_cast(uint32_t, C);
+                const h2 Lp = __builtin_bit_cast(h2, dev::lane_prev(cw));
+                const h2 Rp = __builtin_bit_cast(h2, dev::lane_next(cw));
+                const h2 Up = (j > 0) ? w[j > 0 ? j - 1 : 0][t - 1][1]
+                                      : __builtin_bit_cast(h2, sh.v[t - 1][prev][r0][tx + 1]);
+                const h2 Dp = (j + 1 < R) ? w[j + 1 < R ? j + 1 : 0][t - 1][1]
+                                          : __builtin_bit_cast(h2, sh.v[t - 1][prev][r0 + R + 1][tx + 1]);
+                const h2 xm = {Lp.y, C.x};
+                const h2 xp = {C.y, Rp.x};
+                // Tree order, the z+ value (level t-1 of this iteration) last:
+                // the chain through the levels is one add and one fma per level.
+                const h2 part = ((xm + xp) + (Up + Dp)) + w[j][t - 1][0];
+                const h2 sum = part + w[j][t - 1][2];
+                newv[j][t] = __builtin_elementwise_fma(iq[j][t], sum, -qq[j][t]);
+            }
+            if (t < NL) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    w[j][t][0] = w[j][t][1];
+                    w[j][t][1] = w[j][t][2];
+                    w[j][t][2] = newv[j][t];
+                }
+            }
+        }
+        const int zo = k - T;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (NL == T && writes[j] && zo >= z0 && zo < z1)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, newv[j][NL]), ro,
+                                                      (int)(o4[j] + (uint32_t)(zo - plo) * wplane * 4u), 0, 0);
+            // The scaled right-hand side for the later launches of the
+            // refinement, stored as its plane is consumed: a store issued
+            // among the prefetch loads would make every later wait drain all
+            // outstanding loads.
+            if (FIRST && writes[j] && k >= z0 && k < z1)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, qv[j]), rq,
+                                                      (int)(o4[j] + (uint32_t)(k - plo) * wplane * 4u), 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < T && t <= NL; ++t)
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                sh.v[t][cur][r0 + j + 1][tx + 1] = __builtin_bit_cast(uint32_t, newv[j][t]);
+        __syncthreads();
+    }
+}
+
+template <int TXP, int TY, int T, bool FIRST, int R, int K>
+__device__ __forceinline__ void ref_sweep_dispatch(int nl, const Grid& g, const uint32_t* __restrict__ ein,
+                                                   uint32_t* __restrict__ eout,
+                                                   const float* __restrict__ q,
+                                                   uint32_t* __restrict__ qi,
+                                                   const uint16_t* __restrict__ fl2, float s, int zc,
+                                                   int tile_x, int tile_y, int tile_z,
+                                                   HalfLds<TXP, TY, T>& sh)
+{
+    if constexpr (K == 0) {
+        ref_sweep_levels<TXP, TY, T, FIRST, R, 0>(g, ein, eout, q, qi, fl2, s, zc, tile_x, tile_y,
+                                                  tile_z, sh);
+    } else {
+        if (nl == K)
+            ref_sweep_levels<TXP, TY, T, FIRST, R, K>(g, ein, eout, q, qi, fl2, s, zc, tile_x,
+                                                      tile_y, tile_z, sh);
+        else
+            ref_sweep_dispatch<TXP, TY, T, FIRST, R, K - 1>(nl, g, ein, eout, q, qi, fl2, s, zc,
+                                                            tile_x, tile_y, tile_z, sh);
+    }
+}
+
+// One tile per block of TXP x (TY / R) threads.  Requires zc <=
+// fused_max_zc(plane) (32-bit buffer offsets).
+template <int TXP, int TY, int T, bool FIRST, int R>
+__global__ __launch_bounds__(TXP * TY / R, 8 / R) void k_ref_sweeps(Grid g, const uint32_t* __restrict__ ein,
+                                                                 uint32_t* __restrict__ eout,
+                                                                 const float* __restrict__ q,
+                                                                 uint32_t* __restrict__ qi,
+                                                                 const uint16_t* __restrict__ fl2,
+                                                                 const RefineCtl* __restrict__ ctl, int zc)
 {
     static_assert(T % 2 == 0, "pairs: the halo must be whole pairs");
+    static_assert(R >= 1 && TY % R == 0, "whole tile rows per thread");
+    static_assert(64 % TXP == 0, "a wave holds whole thread rows");
     if (ctl->done) return;  // uniform across the grid
     __shared__ HalfLds<TXP, TY, T> sh;
+    constexpr int NT = TXP * TY / R;  // threads
     {
         uint32_t* flat = &sh.v[0][0][0][0];
         const int n = T * 2 * (TY + 2) * (TXP + 2);
-        for (int k = (int)(threadIdx.x + TXP * threadIdx.y); k < n; k += TXP * TY) flat[k] = 0u;
-        for (int f = (int)(threadIdx.x + TXP * threadIdx.y); f < 256; f += TXP * TY)
+        for (int k = (int)(threadIdx.x + TXP * threadIdx.y); k < n; k += NT) flat[k] = 0u;
+        for (int f = (int)(threadIdx.x + TXP * threadIdx.y); f < 256; f += NT)
             sh.itab[f] = __builtin_bit_cast(uint16_t, dev::to_f16(dev::cell_inv((uint8_t)f)));
     }
     __syncthreads();
@@ -332,129 +544,13 @@ __global__ __launch_bounds__(TXP * TY, 8) void k_ref_sweeps(Grid g, const uint32
     const int full = nb & ~7;
     const int L = (b < full) ? (b & 7) * (full >> 3) + (b >> 3) : b;
     const int tile_x = L % nbx, tile_y = (L / nbx) % nby, tile_z = L / (nbx * nby);
-
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    constexpr int OXC = 2 * TXP - 2 * T, OY = TY - 2 * T;
-    const int x0 = tile_x * OXC + 2 * tx - T;
-    const int y = tile_y * OY + ty - T + 1;
-    const int z0 = tile_z * zc + 1;
-    const int z1 = min(z0 + zc, g.nz - 1);
-    const bool in_grid = x0 >= 0 && x0 < g.px && y >= 0 && y <= g.ny;
-    const bool writes = tx >= T / 2 && tx < TXP - T / 2 && ty >= T && ty < TY - T && x0 >= 0 &&
-                        x0 < g.px && y >= 1 && y <= g.ny - 2;
-    const int wbase = in_grid ? (g.idx(x0, y, 0) >> 1) : 0;
-    const int wplane = g.plane >> 1;
     const float s = FIRST ? ctl->s : 0.0f;
-
-    const h2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
-    h2 w[T][3];
-    h2 iq[T + 1], qq[T + 1];
-#pragma unroll
-    for (int t = 0; t < T; ++t) w[t][0] = w[t][1] = w[t][2] = zero2;
-#pragma unroll
-    for (int t = 0; t <= T; ++t) iq[t] = qq[t] = zero2;
-
-    // Prefetch ring (plane k + j in slot j), dev::kPrefetch planes deep.  The
-    // ring holds loaded values only (FIRST: the raw FP32 r); anything computed
-    // from them is computed when the plane is consumed, so no wait is forced
-    // at the load.
-    constexpr int D = dev::kPrefetch;
-    h2 ne[D], nq[D];
-    uint16_t nfl[D];  // the pair's two flags bytes
-    float2 nr[D];
-    // Loads are unconditional (plane index clamped into the field, lanes
-    // outside the grid read row 0); planes outside [0, nz] or past the chunk
-    // read as 0 when consumed.
-    auto load = [&](int k, h2& e_, uint16_t& f_, h2& q_, float2& r_) {
-        const int wi = wbase + min(max(k, 0), g.nz) * wplane;
-        f_ = fl2[wi];
-        if (FIRST) {
-            e_ = zero2;
-            q_ = zero2;
-            r_ = *reinterpret_cast<const float2*>(q + 2 * wi);
-        } else {
-            e_ = ld_h2(ein, wi);
-            q_ = ld_h2(qi, wi);
-            r_ = make_float2(0.0f, 0.0f);
-        }
-    };
-    auto valid = [&](int k) { return in_grid && k >= 0 && k <= g.nz && k < z1 + T; };
-#pragma unroll
-    for (int j = 0; j < D; ++j) load(z0 - T + j, ne[j], nfl[j], nq[j], nr[j]);
-    // Unrolled by lcm(3, T+1) so the register windows rotate without copies;
-    // the padded iterations load and write nothing.
-    constexpr int U = dev::z_unroll(T);
-    const int kend = z0 - T + ((z1 - z0 + 2 * T + U - 1) / U) * U;
-    for (int kb = z0 - T; kb < kend; kb += U)
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int k = kb + u;
-        const bool ok = valid(k);
-        const uint16_t fp = ok ? nfl[0] : (uint16_t)0;  // flags 0: inv = 0
-        const h2 ii = {__builtin_bit_cast(_Float16, sh.itab[fp & 0xff]),
-                       __builtin_bit_cast(_Float16, sh.itab[fp >> 8])};
-        const h2 p0 = ok ? ne[0] : zero2;
-        h2 qv = ok ? nq[0] : zero2;
-        if (FIRST) {
-            const float2 rv = ok ? nr[0] : make_float2(0.0f, 0.0f);
-            qv.x = dev::to_f16(rv.x * s * (float)ii.x);
-            qv.y = dev::to_f16(rv.y * s * (float)ii.y);
-        }
-#pragma unroll
-        for (int j = 0; j + 1 < D; ++j) {
-            ne[j] = ne[j + 1];
-            nfl[j] = nfl[j + 1];
-            nq[j] = nq[j + 1];
-            nr[j] = nr[j + 1];
-        }
-        load(k + D, ne[D - 1], nfl[D - 1], nq[D - 1], nr[D - 1]);
-#pragma unroll
-        for (int t = T; t > 0; --t) {
-            iq[t] = iq[t - 1];
-            qq[t] = qq[t - 1];
-        }
-        iq[0] = ii;
-        qq[0] = qv;
-        w[0][0] = w[0][1];
-        w[0][1] = w[0][2];
-        w[0][2] = p0;
-        const int cur = k & 1, prev = cur ^ 1;
-        h2 newv[T + 1];
-        newv[0] = p0;
-#pragma unroll
-        for (int t = 1; t <= T; ++t) {
-            // x neighbours from the adjacent lanes (DPP), y neighbours from LDS.
-            const h2 C = w[t - 1][1];
-            const uint32_t cw = __builtin_bit_cast(uint32_t, C);
-            const h2 Lp = __builtin_bit_cast(h2, dev::lane_prev(cw));
-            const h2 Rp = __builtin_bit_cast(h2, dev::lane_next(cw));
-            const h2 Up = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty][tx + 1]);
-            const h2 Dp = __builtin_bit_cast(h2, sh.v[t - 1][prev][ty + 2][tx + 1]);
-            const h2 xm = {Lp.y, C.x};
-            const h2 xp = {C.y, Rp.x};
-            // Tree order, the z+ value (level t-1 of this iteration) last:
-            // the chain through the levels is one add and one fma per level.
-            const h2 part = ((xm + xp) + (Up + Dp)) + w[t - 1][0];
-            const h2 sum = part + w[t - 1][2];
-            const h2 v = __builtin_elementwise_fma(iq[t], sum, -qq[t]);
-            newv[t] = v;
-            if (t < T) {
-                w[t][0] = w[t][1];
-                w[t][1] = w[t][2];
-                w[t][2] = v;
-            }
-        }
-        const int zo = k - T;
-        if (writes && zo >= z0 && zo < z1) st_h2(eout, wbase + zo * wplane, newv[T]);
-        // The scaled right-hand side for the later launches of the refinement,
-        // stored as its plane is consumed: a store issued among the prefetch
-        // loads would make every later wait drain all outstanding loads.
-        if (FIRST && writes && k >= z0 && k < z1) st_h2(qi, wbase + k * wplane, qv);
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-            sh.v[t][cur][ty + 1][tx + 1] = __builtin_bit_cast(uint32_t, newv[t]);
-        __syncthreads();
-    }
+    // Levels the wave needs: its tile rows r0w .. r0w + RPW - 1.
+    constexpr int TPW = 64 / TXP, RPW = TPW * R;
+    const int r0w = __builtin_amdgcn_readfirstlane((int)threadIdx.y / TPW * TPW) * R;
+    const int nl = min(T, min(r0w + RPW - 1, TY - 1 - r0w));
+    ref_sweep_dispatch<TXP, TY, T, FIRST, R, T>(nl, g, ein, eout, q, qi, fl2, s, zc, tile_x, tile_y,
+                                                tile_z, sh);
 }
 
 inline dim3 resid_grid(const Grid& g)
@@ -481,20 +577,37 @@ hipError_t reduce_and_finish(const RefineBufs& b, int n, int stage, double tol, 
     return hipGetLastError();
 }
 
-template <int T, bool FIRST>
+// Tiles of 32 column pairs x 32 rows, one row per thread, or x 64 rows, two
+// rows per thread (1024 threads either way).
+template <int T, bool FIRST, int R>
 hipError_t launch_sweeps(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
                          int ein, int eout, int zc, hipStream_t st)
 {
     // Two flags bytes per 16-bit word: the pair (x0, x0+1) of word x0/2.
     const uint16_t* flags2 = reinterpret_cast<const uint16_t*>(flags);
-    constexpr int TXP = 32, TY = 32;
+    constexpr int TXP = 32, TY = 32 * R;
     constexpr int OXC = 2 * TXP - 2 * T, OY = TY - 2 * T;
     dim3 grid((unsigned)((g.px + OXC - 1) / OXC), (unsigned)((g.ny - 2 + OY - 1) / OY),
               (unsigned)((g.nz - 2 + zc - 1) / zc));
-    hipLaunchKernelGGL((k_ref_sweeps<TXP, TY, T, FIRST>), grid, dim3(TXP, TY), 0, st, g,
+    hipLaunchKernelGGL((k_ref_sweeps<TXP, TY, T, FIRST, R>), grid, dim3(TXP, TY / R), 0, st, g,
                        (const uint32_t*)b.e[ein], b.e[eout], q, b.qi, flags2, b.ctl,
                        zc);
     return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_sweeps_t(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
+                           int first, int ein, int eout, int sweeps, int zc, hipStream_t st)
+{
+    switch (sweeps * 2 + (first ? 1 : 0)) {
+    case 4: return launch_sweeps<2, false, R>(g, b, q, flags, ein, eout, zc, st);
+    case 5: return launch_sweeps<2, true, R>(g, b, q, flags, ein, eout, zc, st);
+    case 8: return launch_sweeps<4, false, R>(g, b, q, flags, ein, eout, zc, st);
+    case 9: return launch_sweeps<4, true, R>(g, b, q, flags, ein, eout, zc, st);
+    case 12: return launch_sweeps<6, false, R>(g, b, q, flags, ein, eout, zc, st);
+    case 13: return launch_sweeps<6, true, R>(g, b, q, flags, ein, eout, zc, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -516,18 +629,15 @@ hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, con
 }
 
 hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
-                         int first, int ein, int eout, int sweeps, int zc, hipStream_t st)
+                         int first, int ein, int eout, int sweeps, int zc, int two_rows,
+                         hipStream_t st)
 {
+    // The z-chunk is a speed parameter: cap it so a block's span of planes
+    // fits the 32-bit buffer offsets (planes too wide for any chunk: error).
+    zc = std::min(zc, fused_max_zc(g.plane));
     if (zc < 1) return hipErrorInvalidValue;
-    switch (sweeps * 2 + (first ? 1 : 0)) {
-    case 4: return launch_sweeps<2, false>(g, b, q, flags, ein, eout, zc, st);
-    case 5: return launch_sweeps<2, true>(g, b, q, flags, ein, eout, zc, st);
-    case 8: return launch_sweeps<4, false>(g, b, q, flags, ein, eout, zc, st);
-    case 9: return launch_sweeps<4, true>(g, b, q, flags, ein, eout, zc, st);
-    case 12: return launch_sweeps<6, false>(g, b, q, flags, ein, eout, zc, st);
-    case 13: return launch_sweeps<6, true>(g, b, q, flags, ein, eout, zc, st);
-    default: return hipErrorInvalidValue;
-    }
+    return two_rows ? launch_sweeps_t<2>(g, b, q, flags, first, ein, eout, sweeps, zc, st)
+                    : launch_sweeps_t<1>(g, b, q, flags, first, ein, eout, sweeps, zc, st);
 }
 
 hipError_t refine_resid(const Grid& g, const float* xo, float* xn, const float* d,

@@ -349,6 +349,8 @@ hipError_t enqueue_refined(smoke_sim* s, long long* launches, long long* sweep_l
     const int R = (p.iterations + p.inner_sweeps - 1) / p.inner_sweeps;
     int T = s->plan.sweeps_per_launch;
     if (T < 2 || T > 6 || (T & 1)) T = 4;
+    const int two_rows = (s->plan.tile == smk::JacobiPlan::TILE_64x32 ||
+                          s->plan.tile == smk::JacobiPlan::TILE_32x64) ? 1 : 0;
     hipError_t e = smk::refine_norm0(g, s->d, s->flags, b, p.tolerance, s->stream);
     if (e != hipSuccess) return e;
     *launches += 2;
@@ -360,7 +362,7 @@ hipError_t enqueue_refined(smoke_sim* s, long long* launches, long long* sweep_l
             const int t = std::min(left, T);
             const int eout = first ? 0 : 1 - ein;
             e = smk::refine_sweeps(g, b, r == 0 ? s->d : b.q, s->flags, first ? 1 : 0, ein, eout, t,
-                                   s->plan.zc, s->stream);
+                                   s->plan.zc, two_rows, s->stream);
             if (e != hipSuccess) return e;
             ein = eout;
             first = false;
@@ -1084,28 +1086,61 @@ smoke_status smoke_autotune(smoke_sim* s, smoke_jacobi_plan* chosen, double* ms_
     }
     constexpr int kReps = 2;
     hipError_t e = hipSuccess;
-    for (int t = 2; t <= 6 && e == hipSuccess; ++t)
-        for (int tile : tiles)
+    // REFINED: the refined solve's sequence (FP16 sweep launches of an even
+    // T, one- or two-row tiles, and the FP32 residual passes), per FP16 sweep.
+    const bool refined = s->prm.projection == SMOKE_PROJ_REFINED;
+    const int pcur0 = s->pcur;
+    const smk::JacobiPlan plan0 = s->plan;
+    const long long fp16_sweeps =
+        (long long)((s->prm.iterations + s->prm.inner_sweeps - 1) / s->prm.inner_sweeps) *
+        s->prm.inner_sweeps;
+    if (refined) {
+        // A right-hand side to solve: the divergence of the last output
+        // velocity, half 1 - now (a zero one would end every refined solve at
+        // its first norm).  d then no longer holds the last projection's
+        // right-hand side.
+        const int h = 1 - s->now;
+        e = smk::with_fields(s, [&](auto, auto u, auto v, auto w) {
+            return smk::divergence(s->g, u[h], v[h], w[h], s->flags, s->d, s->stream);
+        });
+        s->rhs_valid = false;
+    }
+    for (int t = 2; t <= 6 && e == hipSuccess; ++t) {
+        if (refined && (t & 1)) continue;
+        for (int tile : tiles) {
+            if (refined && tile != smk::JacobiPlan::TILE_32x32 && tile != smk::JacobiPlan::TILE_64x32)
+                continue;  // the refined sweeps have one tile per row count
             for (int zc : zcs) {
                 smk::JacobiPlan cand;
                 cand.sweeps_per_launch = t;
                 cand.tile = tile;
                 cand.zc = zc;
-                long long n = 0;
-                e = enqueue_jacobi(s, cand, K, false, &n);  // warm-up
+                long long n = 0, n2 = 0;
+                auto run = [&]() {
+                    if (!refined) return enqueue_jacobi(s, cand, K, false, &n);
+                    s->plan = cand;
+                    const hipError_t r = enqueue_refined(s, &n, &n2);
+                    s->pcur = pcur0;
+                    return r;
+                };
+                e = run();  // warm-up
                 if (e == hipSuccess) e = hipEventRecord(ev[0], s->stream);
-                for (int r = 0; r < kReps && e == hipSuccess; ++r) e = enqueue_jacobi(s, cand, K, false, &n);
+                for (int r = 0; r < kReps && e == hipSuccess; ++r) e = run();
                 if (e == hipSuccess) e = hipEventRecord(ev[1], s->stream);
                 if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
                 float ms = 0.0f;
                 if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
                 if (e != hipSuccess) break;
-                const double per_sweep = ms / (double)(kReps * K);
+                const double per_sweep = ms / (double)(kReps * (refined ? fp16_sweeps : K));
                 if (best_ms < 0.0 || per_sweep < best_ms) {
                     best_ms = per_sweep;
                     best = cand;
                 }
             }
+        }
+    }
+    s->plan = plan0;
+    s->pcur = pcur0;
     // The sweeps ran on whatever right-hand side the buffer held: start the
     // next warm start from zero.
     if (e == hipSuccess) e = smk::fill_f32(s->P[0], s->g.elems, 0.0f, s->stream);

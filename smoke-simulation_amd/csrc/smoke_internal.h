@@ -364,9 +364,12 @@ int refine_parts(const Grid& g);
 hipError_t refine_norm0(const Grid& g, const float* d, const uint8_t* flags, const RefineBufs& b,
                         double tol, hipStream_t st);
 // `sweeps` (2, 4 or 6) FP16 sweeps e[ein] -> e[eout]; `first`: e starts at 0
-// and the scaled right-hand side is formed from the FP32 q.
+// and the scaled right-hand side is formed from the FP32 q.  two_rows: tiles
+// of 64 rows with two rows per thread (else 32 rows, one per thread); every
+// choice gives the same bits.
 hipError_t refine_sweeps(const Grid& g, const RefineBufs& b, const float* q, const uint8_t* flags,
-                         int first, int ein, int eout, int sweeps, int zc, hipStream_t st);
+                         int first, int ein, int eout, int sweeps, int zc, int two_rows,
+                         hipStream_t st);
 // x_new = x_old + e[ein]/s and the next residual (norms, control update for
 // refinement `stage`).
 hipError_t refine_resid(const Grid& g, const float* xo, float* xn, const float* d,

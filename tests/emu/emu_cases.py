@@ -210,12 +210,13 @@ def case_precision():
 def case_refine_groups():
     # The refined solve's FP16 sweeps give the same bits however they are
     # grouped into fused launches (plans T = 2, 4, 6 split 10 inner sweeps as
-    # 2+2+2+2+2, 4+4+2 or 6+4).
+    # 2+2+2+2+2, 4+4+2 or 6+4) and whatever the tile (one or two rows per
+    # thread).
     sc = scenes.test_scene((21, 13, 11), seed=9, projection=2, iterations=20)
     states = []
-    for t in (2, 4, 6):
+    for t, tile in ((2, 0), (4, 0), (6, 0), (4, 2), (6, 3)):
         g = gpu_for(sc, storage=smokesim.STORAGE_BF16_VELOCITY)
-        g.set_jacobi_plan(sweeps_per_launch=t, tile=0, zc=5)
+        g.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=5)
         g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=10, iterations=20)
         for _ in range(2):
             g.step(sc["dt"], readback=False)

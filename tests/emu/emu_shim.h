@@ -98,6 +98,21 @@ inline uint8_t emu_buf_ld8(EmuRsrc r, int off, int soff)
 {
     return emu_in_range(r, off, soff, 1) ? (uint8_t)r.base[(uint32_t)off + (uint32_t)soff] : (uint8_t)0;
 }
+inline uint16_t emu_buf_ld16(EmuRsrc r, int off, int soff)
+{
+    uint16_t v = 0;
+    if (emu_in_range(r, off, soff, 2)) std::memcpy(&v, r.base + (uint32_t)off + (uint32_t)soff, 2);
+    return v;
+}
+// Two dwords, each range-checked on its own.
+typedef uint32_t emu_u32x2 __attribute__((vector_size(8)));
+inline emu_u32x2 emu_buf_ld64(EmuRsrc r, int off, int soff)
+{
+    emu_u32x2 v;
+    v[0] = emu_buf_ld32(r, off, soff);
+    v[1] = emu_buf_ld32(r, (int)((uint32_t)off + 4u), soff);
+    return v;
+}
 inline void emu_buf_st32(uint32_t v, EmuRsrc r, int off, int soff)
 {
     if (emu_in_range(r, off, soff, 4)) std::memcpy(r.base + (uint32_t)off + (uint32_t)soff, &v, 4);
@@ -105,6 +120,8 @@ inline void emu_buf_st32(uint32_t v, EmuRsrc r, int off, int soff)
 #define __builtin_amdgcn_make_buffer_rsrc(p, stride, bytes, flags) emu_make_rsrc((p), (bytes))
 #define __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, aux) emu_buf_ld32((r), (off), (soff))
 #define __builtin_amdgcn_raw_buffer_load_b8(r, off, soff, aux) emu_buf_ld8((r), (off), (soff))
+#define __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, aux) emu_buf_ld16((r), (off), (soff))
+#define __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, aux) emu_buf_ld64((r), (off), (soff))
 #define __builtin_amdgcn_raw_buffer_store_b32(v, r, off, soff, aux) emu_buf_st32((v), (r), (off), (soff))
 template <typename T>
 T __shfl_xor(T v, int mask, int width = 64)

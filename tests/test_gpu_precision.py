@@ -317,13 +317,14 @@ def test_bf16_velocity_within_tolerance(dims, seed, projection):
 
 @pytest.mark.parametrize("storage", [smokesim.STORAGE_FP32, smokesim.STORAGE_BF16_VELOCITY])
 def test_refined_result_does_not_depend_on_the_launch_grouping(storage):
-    # 20 inner sweeps run as 10 launches of 2, 4+4+4+4+4, or 6+6+6+2: the
-    # same sweeps, the same bits.
+    # 20 inner sweeps run as 10 launches of 2, 4+4+4+4+4, or 6+6+6+2, with
+    # one or two rows per thread: the same sweeps, the same bits.
     sc = scenes.test_scene((97, 70, 45), seed=11, projection=smokesim.PROJ_JACOBI, iterations=60)
     states = []
-    for t in (2, 4, 6):
+    for t, tile in ((2, smokesim.TILE_32x32), (4, smokesim.TILE_32x32), (6, smokesim.TILE_32x32),
+                    (4, smokesim.TILE_64x32), (6, smokesim.TILE_64x32), (2, smokesim.TILE_32x64)):
         g = gpu_for(sc, storage=storage)
-        g.set_jacobi_plan(sweeps_per_launch=t, tile=smokesim.TILE_32x32, zc=16)
+        g.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=16)
         g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=20, iterations=60)
         for _ in range(3):
             g.step(sc["dt"], readback=False)
@@ -337,3 +338,35 @@ def test_refined_result_does_not_depend_on_the_launch_grouping(storage):
                 assert np.array_equal(np.ascontiguousarray(v).view(np.uint8),
                                       np.ascontiguousarray(states[0][1][k]).view(np.uint8)), (t, k)
         assert info == states[0][2], (t, info)
+
+
+def test_autotune_of_the_refined_solve_keeps_its_results():
+    # With REFINED the autotuner times the refined solve: it keeps an even
+    # sweep count and a tile the refined sweeps have, leaves the state as it
+    # was, and the tuned plan gives the bits of a fixed plan.
+    sc = scenes.test_scene((97, 70, 45), seed=12, projection=smokesim.PROJ_JACOBI, iterations=60)
+    states = []
+    for tune in (False, True):
+        g = gpu_for(sc, storage=smokesim.STORAGE_FP16)
+        g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=30, iterations=60)
+        g.set_jacobi_plan(sweeps_per_launch=4, tile=smokesim.TILE_32x32, zc=64)
+        g.step(sc["dt"], readback=False)
+        before = g.state()
+        if tune:
+            plan, ms = g.autotune()
+            assert ms > 0 and plan["sweeps_per_launch"] in (2, 4, 6), plan
+            assert plan["tile"] in (smokesim.TILE_32x32, smokesim.TILE_64x32), plan
+            after = g.state()
+            for k, v in before.items():
+                if isinstance(v, np.ndarray):
+                    assert np.array_equal(v.view(np.uint8), after[k].view(np.uint8)), k
+        for _ in range(2):
+            g.step(sc["dt"], readback=False)
+        st = g.state()
+        st["pressure"] = g.get_pressure()
+        states.append((st, g.solve_info()))
+        g.close()
+    for k, v in states[0][0].items():
+        if isinstance(v, np.ndarray):
+            assert np.array_equal(v.view(np.uint8), states[1][0][k].view(np.uint8)), k
+    assert states[0][1] == states[1][1]
