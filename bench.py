@@ -470,7 +470,18 @@ def run_single(args, sc):
 
     # Whole step: compulsory bytes with the launches the step makes (fixed
     # stages once, one sweep's 13 B per fused launch) and the unfused model.
-    comp_bpc = (BYTES_FIXED + BYTES_SWEEP * launches_per_step) if p["projection"] == 2 else bpc
+    fixed = FIXED_BY_STORAGE[storage]
+    if p["projection"] == 2:
+        comp_bpc = fixed + BYTES_SWEEP * launches_per_step
+    elif p["projection"] == 3:
+        # Fused FP16 launches move the sweep bytes once each.
+        refinements = -(-p["iterations"] // p["inner_sweeps"])
+        t = r["plan"]["sweeps_per_launch"]
+        t = t if 2 <= t <= 6 and t % 2 == 0 else 4
+        comp_bpc = fixed + BYTES_REF_NORM0 + refinements * (
+            -(-p["inner_sweeps"] // t) * BYTES_REF_SWEEP + BYTES_REF_RESID)
+    else:
+        comp_bpc = bpc
     step_comp = cells * comp_bpc / (ms_per_step * 1e-3) / 1e9
     step_eff = cells * bpc / (ms_per_step * 1e-3) / 1e9
     out = {
