@@ -47,7 +47,9 @@ typedef enum smoke_status {
  * solve of that system: an FP32 residual-correction loop whose corrections
  * take `inner_sweeps` FP16 Jacobi sweeps each, FP64 norms on the device; it
  * runs ceil(iterations / inner_sweeps) refinements, fewer once the relative
- * residual reaches `tolerance` (> 0).  Cold start only. */
+ * residual reaches `tolerance` (> 0).  Cold start only; planes of at most
+ * 10956549 cells (nx+1 padded to 32, times ny+1; smoke_set_params refuses
+ * REFINED on wider ones). */
 typedef enum smoke_projection {
     SMOKE_PROJ_REFERENCE = 0,
     SMOKE_PROJ_RBSOR = 1,

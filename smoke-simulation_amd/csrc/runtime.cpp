@@ -185,6 +185,11 @@ namespace {
 // Allocates and clears the REFINED buffers (once per simulation).
 smoke_status ensure_refine(smoke_sim* s)
 {
+    // The FP16 sweeps address a block's planes with 32-bit buffer offsets
+    // (as the FP32 fused sweeps do) and have no one-sweep fallback.
+    if (smk::fused_max_zc(s->g.plane) < 1)
+        return fail(SMOKE_ERR_INVALID, "refined solver: planes of %d cells are too wide (at most %lld)",
+                    s->g.plane, 2147483647LL / (4LL * (smk::kFusedSlackPlanes + 1)));
     if (s->refine_mem) return SMOKE_OK;
     const Grid& g = s->g;
     const size_t bytes = refine_bytes(g);
