@@ -501,6 +501,7 @@ smoke_status smoke_batch_autotune(smoke_batch* b, smoke_jacobi_plan* chosen, dou
     for (int t = 2; t <= 6 && e == hipSuccess; ++t)
         for (int tile : tiles)
             for (int zc : zcs) {
+                if (e != hipSuccess) break;  // a failed candidate ends the search
                 b->plan.sweeps_per_launch = t;
                 b->plan.tile = tile;
                 b->plan.zc = std::min(zc, b->zc_cap);

@@ -339,6 +339,7 @@ __device__ __forceinline__ void ref_sweep_levels(const Grid& g, const uint32_t* 
     const __amdgpu_buffer_rsrc_t rr = dev::buf_rsrc(q + 2 * firstw, FIRST ? span * 8u : 0u);
     const __amdgpu_buffer_rsrc_t ro = dev::buf_rsrc(eout + firstw, span * 4u);
     uint32_t o4[R];  // byte offset of the lane's pair word in plane plo (0x80000000: outside)
+    uint32_t o8[R];  // the same for the FP32 pair of the FIRST launch's residual
     bool writes[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -348,6 +349,7 @@ __device__ __forceinline__ void ref_sweep_levels(const Grid& g, const uint32_t* 
         writes[j] = tx >= T / 2 && tx < TXP - T / 2 && r >= T && r < TY - T && x0 >= 0 &&
                     x0 < g.px && y >= 1 && y <= g.ny - 2;
         o4[j] = in_grid ? (uint32_t)(g.idx(x0, y, 0) >> 1) * 4u : 0x80000000u;
+        o8[j] = in_grid ? (uint32_t)(g.idx(x0, y, 0) >> 1) * 8u : 0x80000000u;
     }
 
     const h2 zero2 = {(_Float16)0.0f, (_Float16)0.0f};
@@ -377,7 +379,7 @@ __device__ __forceinline__ void ref_sweep_levels(const Grid& g, const uint32_t* 
             const uint32_t o = o4[j] + dk * 4u;
             nfl[j][i] = __builtin_amdgcn_raw_buffer_load_b16(rf, (int)(o >> 1), 0, 0);
             if (FIRST) {
-                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, (int)(o << 1), 0, 0);
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, (int)(o8[j] + dk * 8u), 0, 0);
                 // Elements through scalars: a bit cast of v[1] itself reads
                 // element 0 with this compiler.
                 const uint32_t v0 = v[0], v1 = v[1];

@@ -1113,6 +1113,7 @@ smoke_status smoke_autotune(smoke_sim* s, smoke_jacobi_plan* chosen, double* ms_
     for (int t = 2; t <= 6 && e == hipSuccess; ++t) {
         if (refined && (t & 1)) continue;
         for (int tile : tiles) {
+            if (e != hipSuccess) break;
             if (refined && tile != smk::JacobiPlan::TILE_32x32 && tile != smk::JacobiPlan::TILE_64x32)
                 continue;  // the refined sweeps have one tile per row count
             for (int zc : zcs) {
