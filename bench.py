@@ -537,6 +537,16 @@ def run_single(args, sc):
             "plan": m["plan"], "autotune": m["autotune"],
             "step_norms": m["norms"],
         }
+        # The same solve split into five refinements of 20 FP16 sweeps (the
+        # equal sweep budget with more FP32 residual passes).
+        m5 = time_form(args, sc, MIXED_STORAGE, dict(MIXED, inner_sweeps=20, iterations=fp32_iterations),
+                       args_plan=False)
+        m5ms = 1000.0 * m5["wall"] / args.steps
+        out["mixed"]["five_refinements"] = {
+            "inner_sweeps": 20, "ms_per_step": round(m5ms, 4),
+            "step_rate_vs_fp32": round(ms_per_step / m5ms, 4), "plan": m5["plan"],
+            "solve": ({"relative_residual": m5["solve"][0], "refinements": m5["solve"][1]}
+                      if m5["solve"] else None)}
     if args.error_steps:
         esc = dict(sc, params=dict(sc["params"], projection=2, iterations=fp32_iterations))
         out["accuracy"] = error_check(esc, args.error_steps, fp32_iterations)
