@@ -76,3 +76,30 @@ def test_batched_step_equals_fp32_fixture(use_graph):
             assert g.index_now == int(fx["now"])
             out = outputs(g, g.index_now)
     check_fp32_bitwise(out, fx, f"batch graph={use_graph}")
+
+
+@pytest.mark.parametrize("name,storage,refined", [
+    ("p1_16x12x20", smokesim.STORAGE_FP16, False),
+    ("p2_20x16x12", smokesim.STORAGE_FP16, False),
+    ("p2_20x16x12", smokesim.STORAGE_FP32, True),
+    ("p2_20x16x12", smokesim.STORAGE_FP16, True)],
+    ids=["fp16-rbsor", "fp16-jacobi", "fp32-refined", "fp16-refined"])
+def test_reduced_precision_forms_meet_tolerance_of_fp64_fixture(name, storage, refined):
+    """The 16-bit storage forms and the refined solve against the stored FP64
+    outputs: mask and flags bit-exact; relative L2 of density <= 1e-3 and of
+    velocity and pressure <= 3e-3 (the refined solve runs 2 refinements of
+    half the Jacobi sweeps, the same sweep count)."""
+    fx = mf.load_fixture(name)
+    with gpu_from_fixture(fx, storage=storage) as g:
+        if refined:
+            its = json_params(fx)["iterations"]
+            g.set_params(projection=smokesim.PROJ_REFINED, inner_sweeps=its // 2, iterations=its)
+        for _ in range(int(fx["steps"])):
+            g.step(float(fx["dt"]), readback=False)
+        g.synchronize()
+        out = outputs(g, g.index_now)
+    assert np.array_equal(out["mask"], fx["mask"]) and np.array_equal(out["flags"], fx["flags"])
+    errs = {k: rel_l2(out[k], fx[f"{k}64"]) for k in ("rho", "u", "v", "w", "pressure")}
+    print(name, storage, refined, errs)
+    assert errs["rho"] <= 1e-3, errs
+    assert max(errs["u"], errs["v"], errs["w"], errs["pressure"]) <= 3e-3, errs
