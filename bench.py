@@ -192,6 +192,23 @@ def reference_80_rates(args):
                        "(40,8,40), red-black SOR x40 on faces, dt=0.05, 100 steps",
            "gpu": {"value": round(cells / (float(sum(ms)) / len(ms) * 1e-3), 1),
                    "unit": "cell-updates/s", "latency_ms": latency(ms), "steps": len(ms)}}
+    # Host clock over the 100 steps enqueued back to back, one launch per
+    # kernel (eager) or one graph replay per step (smoke_set_step_graph).
+    for name, graph in (("gpu_eager_wall", False), ("gpu_graph_wall", True)):
+        sim = smokesim.Simulation(nx, ny, nz)
+        scenes.apply(sim, sc)
+        sim.set_step_graph(graph)
+        for _ in range(5):
+            sim.step_async(sc["dt"])
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(sc["steps"]):
+            sim.step_async(sc["dt"])
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        sim.close()
+        out[name] = {"value": round(cells * sc["steps"] / wall, 1), "unit": "cell-updates/s",
+                     "ms_per_step": round(1000.0 * wall / sc["steps"], 4)}
     if not args.no_cpu_baseline:
         info = host_info()
         threads = max(1, min(16, info["usable_cores"] or 1))

@@ -273,6 +273,16 @@ smoke_status smoke_step(smoke_sim* sim, float dt, float* density_out);
 /* Enqueues one full step on the simulation's stream and returns. */
 smoke_status smoke_step_async(smoke_sim* sim, float dt);
 
+/* Step graphs (off by default): with enable != 0, smoke_step / _async capture
+ * a step into a HIP graph, one per ping-pong parity, and replay it -- one
+ * launch per step instead of one per kernel, for grids small enough that
+ * launches bound the step.  Same bits as eager steps.  A graph is captured
+ * again after any change of parameters, plan or objects, or of dt; the mask
+ * refresh after an obstacle or solid change runs before the replay.  Steps
+ * with timing or step norms enabled run eagerly.  No reference counterpart
+ * (the reference launches every kernel, smokeSimulation.cu:774-819). */
+smoke_status smoke_set_step_graph(smoke_sim* sim, int enable);
+
 /* Enqueues one stage (smoke_stage) of a step; five stages in order equal one
  * step. */
 smoke_status smoke_run_stage(smoke_sim* sim, int stage, float dt);

@@ -578,10 +578,76 @@ hipError_t collect_timing(smoke_sim* s, bool wait)
     return hipSuccess;
 }
 
+void drop_step_graphs(smoke_sim* s)
+{
+    for (smoke_sim::StepGraph& sg : s->graphs) {
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+        if (sg.graph) (void)hipGraphDestroy(sg.graph);
+        sg = smoke_sim::StepGraph{};
+    }
+}
+
+// One step as a replay of the graph captured for the current parity (captured
+// first if there is none for this version, dt and pressure parity).  The mask
+// refresh, when due, runs before it, outside the graph.
+smoke_status step_graph(smoke_sim* s, float dt)
+{
+    HIP_TRY(refresh_mask(s, nullptr));
+    smoke_sim::StepGraph& sg = s->graphs[s->now];
+    if (!sg.exec || sg.version != s->version || sg.dt != dt || sg.pcur_in != s->pcur) {
+        if (sg.exec) {
+            HIP_TRY(hipStreamSynchronize(s->stream));  // no replay of it still running
+            (void)hipGraphExecDestroy(sg.exec);
+        }
+        if (sg.graph) (void)hipGraphDestroy(sg.graph);
+        sg = smoke_sim::StepGraph{};
+        const int now0 = s->now, pcur0 = s->pcur;
+        const bool rhs0 = s->rhs_valid;
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int st = 0; st < SMOKE_STAGE_COUNT && e == hipSuccess; ++st) {
+            long long launches = 0;
+            e = run_stage(s, st, dt, &launches);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+        sg.now_out = s->now;
+        sg.pcur_out = s->pcur;
+        sg.rhs_out = s->rhs_valid;
+        s->now = now0;  // the replay below applies the step's parities
+        s->pcur = pcur0;
+        s->rhs_valid = rhs0;
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            sg = smoke_sim::StepGraph{};
+            return fail(SMOKE_ERR_HIP, "step: graph capture failed: %s",
+                        hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        sg.graph = g;
+        const hipError_t e3 = hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            sg = smoke_sim::StepGraph{};
+            return fail(SMOKE_ERR_HIP, "step: graph instantiation failed: %s", hipGetErrorString(e3));
+        }
+        sg.version = s->version;
+        sg.dt = dt;
+        sg.pcur_in = pcur0;
+    }
+    HIP_TRY(hipGraphLaunch(sg.exec, s->stream));
+    s->now = sg.now_out;
+    s->pcur = sg.pcur_out;
+    s->rhs_valid = sg.rhs_out;
+    s->last_dt = dt;
+    s->steps_enqueued += 1;
+    return SMOKE_OK;
+}
+
 smoke_status step_locked(smoke_sim* s, float dt)
 {
     SMOKE_REFUSE_IN_BATCH(s, "step");
     if (!std::isfinite(dt)) return fail(SMOKE_ERR_INVALID, "step: dt must be finite");
+    if (s->graph_on && !s->timing && !s->norms_on) return step_graph(s, dt);
     smoke_sim::StepEvents se{};
     if (s->timing) {
         if (s->pending.size() >= 512) HIP_TRY(collect_timing(s, false));
@@ -781,6 +847,7 @@ void smoke_destroy(smoke_sim* s)
         (void)hipEventDestroy(se.sweep_ev[1]);
     }
     for (hipEvent_t ev : s->event_pool) (void)hipEventDestroy(ev);
+    drop_step_graphs(s);
     if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
     hipEvent_t rb_events[5] = {s->rb_step_done, s->rb_packed[0], s->rb_packed[1], s->rb_copied[0],
                                s->rb_copied[1]};
@@ -906,6 +973,7 @@ smoke_status smoke_set_params(smoke_sim* s, const smoke_params* p)
         if (os != SMOKE_OK) return os;
     }
     s->prm = *p;
+    s->version += 1;
     return SMOKE_OK;
 }
 
@@ -1017,6 +1085,7 @@ smoke_status smoke_restore(smoke_sim* s, const void* host, size_t bytes)
     s->plan.sweeps_per_launch = h.plan[0];
     s->plan.tile = h.plan[1];
     s->plan.zc = h.plan[2];
+    s->version += 1;
     s->mask_dirty = true;  // flags (and the refined solver's inverse) follow the restored masks
     HIP_TRY(refresh_mask(s, nullptr));
     HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1049,6 +1118,7 @@ smoke_status smoke_set_jacobi_plan(smoke_sim* s, const smoke_jacobi_plan* plan)
     s->plan.sweeps_per_launch = plan->sweeps_per_launch;
     s->plan.tile = plan->tile;
     s->plan.zc = plan->zc;
+    s->version += 1;
     return SMOKE_OK;
 }
 
@@ -1155,6 +1225,7 @@ smoke_status smoke_autotune(smoke_sim* s, smoke_jacobi_plan* chosen, double* ms_
     cleanup();
     if (e != hipSuccess) return fail(SMOKE_ERR_HIP, "autotune: %s", hipGetErrorString(e));
     s->plan = best;
+    s->version += 1;
     if (chosen) {
         chosen->sweeps_per_launch = best.sweeps_per_launch;
         chosen->tile = best.tile;
@@ -1186,6 +1257,7 @@ static smoke_status add_object(smoke_sim* s, int type, float x, float y, float z
     o.r = r;
     o.type = type;
     s->objects.push_back(o);
+    s->version += 1;
     if (type == 0) s->mask_dirty = true;
     if (id) *id = (int)s->objects.size() - 1;
     return SMOKE_OK;
@@ -1211,6 +1283,7 @@ smoke_status smoke_move_object(smoke_sim* s, int id, float x, float y, float z)
         return fail(SMOKE_ERR_INVALID, "move_object: no object with id %d", id);
     Object& o = s->objects[id];
     o.pos[0] = x; o.pos[1] = y; o.pos[2] = z;
+    s->version += 1;
     if (o.type == 0) s->mask_dirty = true;
     return SMOKE_OK;
 }
@@ -1226,6 +1299,7 @@ smoke_status smoke_set_object_velocity(smoke_sim* s, int id, float vx, float vy,
         return fail(SMOKE_ERR_INVALID, "set_object_velocity: no object with id %d", id);
     Object& o = s->objects[id];
     o.vel[0] = vx; o.vel[1] = vy; o.vel[2] = vz;
+    s->version += 1;
     return SMOKE_OK;
 }
 
@@ -1333,6 +1407,18 @@ smoke_status smoke_get_pressure(smoke_sim* s, float* dst)
 }
 
 int smoke_index_now(const smoke_sim* s) { return s ? s->now : -1; }
+
+smoke_status smoke_set_step_graph(smoke_sim* s, int enable)
+{
+    if (!s) return fail(SMOKE_ERR_INVALID, "set_step_graph: NULL sim");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!enable && s->graph_on) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        drop_step_graphs(s);
+    }
+    s->graph_on = enable != 0;
+    return SMOKE_OK;
+}
 
 smoke_status smoke_step_async(smoke_sim* s, float dt)
 {

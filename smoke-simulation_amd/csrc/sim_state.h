@@ -105,6 +105,23 @@ struct smoke_sim {
     std::deque<Readback> rb_pending;
     int rb_next = 0;
 
+    // Step graphs (smoke_set_step_graph): a step captured into a HIP graph
+    // per ping-pong parity and replayed.  `version` counts the changes that
+    // alter what a step enqueues (parameters, plan, objects, solids); a graph
+    // captured at another version, dt or pressure parity is captured again.
+    // The host-side parities a captured step leaves are kept with its graph.
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long version = 0;
+        float dt = 0.0f;
+        int pcur_in = -1, now_out = 0, pcur_out = 0;
+        bool rhs_out = false;
+    };
+    bool graph_on = false;
+    unsigned long long version = 1;
+    StepGraph graphs[2];
+
     // Timing: events at every stage boundary of a step, plus two around the
     // projection's sweep launches (the step's dominant kernel).
     bool timing = false;

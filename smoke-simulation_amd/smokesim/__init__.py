@@ -142,6 +142,7 @@ def _declare(lib):
         "smoke_index_now": (i, [vp]),
         "smoke_step": (st, [vp, f, fp]),
         "smoke_step_async": (st, [vp, f]),
+        "smoke_set_step_graph": (st, [vp, ctypes.c_int]),
         "smoke_run_stage": (st, [vp, i, f]),
         "smoke_read_density": (st, [vp, fp]),
         "smoke_synchronize": (st, [vp]),
@@ -408,6 +409,10 @@ class Simulation:
 
     def step_async(self, dt):
         _check(self.lib.smoke_step_async(self.h, float(dt)))
+
+    def set_step_graph(self, on=True):
+        """Capture steps into a HIP graph per parity and replay them."""
+        _check(self.lib.smoke_set_step_graph(self.h, 1 if on else 0))
 
     def run_stage(self, stage, dt):
         _check(self.lib.smoke_run_stage(self.h, int(stage), float(dt)))

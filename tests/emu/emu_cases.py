@@ -140,6 +140,36 @@ def case_server():
     srv.close()
 
 
+def case_step_graph():
+    # Steps replayed from a captured graph give the eager bits, through
+    # changes that force a new capture (moved objects, parameters, dt).
+    for proj, its, plan in ((0, 3, None), (2, 5, (2, 0, 4)), (3, 8, (4, 2, 16))):
+        sc = scenes.test_scene((14, 12, 10), seed=34, projection=proj, iterations=its,
+                               n_sources=1, n_obstacles=1)
+        if proj == 3:
+            sc["params"]["inner_sweeps"] = 4
+        sims = [gpu_for(sc), gpu_for(sc)]
+        sims[1].set_step_graph(True)
+        if plan:
+            for g in sims:
+                g.set_jacobi_plan(sweeps_per_launch=plan[0], tile=plan[1], zc=plan[2])
+        dt = sc["dt"]
+        for k in range(5):
+            if k == 2:
+                for g in sims:
+                    g.move_object(0, 7.0, 5.0, 4.0)
+                    g.move_object(1, 6.0, 3.0, 5.0)
+            if k == 3:
+                for g in sims:
+                    g.set_params(iterations=its + 2)
+                dt = sc["dt"] * 0.5
+            outs = [g.step(dt) for g in sims]
+            same_bits(outs[1], outs[0], f"graph step {k} projection {proj}")
+        assert_bitwise_equal(sims[1].state(), sims[0].state(), f"graph projection {proj}")
+        for g in sims:
+            g.close()
+
+
 def case_readback():
     sc = scenes.test_scene((13, 12, 11), seed=5, projection=2, iterations=5)
     g = gpu_for(sc)
@@ -293,7 +323,7 @@ def case_norms():
     g.close()
 
 
-CASES = {"proxy": case_proxy, "refine_groups": case_refine_groups, "single": case_single, "norms": case_norms, "plans": case_plans, "batch": case_batch,
+CASES = {"step_graph": case_step_graph, "proxy": case_proxy, "refine_groups": case_refine_groups, "single": case_single, "norms": case_norms, "plans": case_plans, "batch": case_batch,
          "server": case_server, "readback": case_readback, "precision": case_precision,
          "vorticity": case_vorticity, "advection": case_advection}
 

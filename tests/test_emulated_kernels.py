@@ -41,7 +41,7 @@ def emu_lib(tmp_path_factory):
 
 
 @pytest.mark.parametrize("case", ["single", "norms", "plans", "batch", "server", "readback", "vorticity",
-                                  "advection", "refine_groups", "proxy"])
+                                  "advection", "refine_groups", "proxy", "step_graph"])
 def test_emulated_case_matches_oracle(emu_lib, case):
     lib, asan = emu_lib
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0")
