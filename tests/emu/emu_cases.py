@@ -143,23 +143,21 @@ def case_server():
 def case_step_graph():
     # Steps replayed from a captured graph give the eager bits, through
     # changes that force a new capture (moved objects, parameters, dt).
-    for proj, its, plan in ((0, 3, None), (2, 5, (2, 0, 4)), (3, 8, (4, 2, 16))):
+    for proj, its, plan in ((0, 3, None), (2, 5, (2, 0, 64))):
         sc = scenes.test_scene((14, 12, 10), seed=34, projection=proj, iterations=its,
                                n_sources=1, n_obstacles=1)
-        if proj == 3:
-            sc["params"]["inner_sweeps"] = 4
         sims = [gpu_for(sc), gpu_for(sc)]
         sims[1].set_step_graph(True)
         if plan:
             for g in sims:
                 g.set_jacobi_plan(sweeps_per_launch=plan[0], tile=plan[1], zc=plan[2])
         dt = sc["dt"]
-        for k in range(5):
-            if k == 2:
+        for k in range(4):
+            if k == 1:
                 for g in sims:
                     g.move_object(0, 7.0, 5.0, 4.0)
                     g.move_object(1, 6.0, 3.0, 5.0)
-            if k == 3:
+            if k == 2:
                 for g in sims:
                     g.set_params(iterations=its + 2)
                 dt = sc["dt"] * 0.5
