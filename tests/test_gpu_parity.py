@@ -203,3 +203,25 @@ def test_autotune_picks_a_plan_and_keeps_results_bitwise():
         g.step(sc["dt"], readback=False)
         o.step(sc["dt"])
     assert_bitwise_equal(g.state(), o.state(), f"autotuned plan {plan}")
+
+
+@pytest.mark.parametrize("dims", [(3, 3, 3), (4, 5, 3), (65, 9, 12), (9, 70, 6), (130, 33, 5)],
+                         ids=lambda d: f"{d[0]}x{d[1]}x{d[2]}")
+def test_two_row_tiles_on_small_and_thin_grids(dims):
+    """The two-row tiles (one block covers 64x32 or 32x64 cells) on grids
+    smaller or thinner than a tile, every fused sweep count, z-chunks of 1
+    plane and of the whole depth: the FP32 oracle's bits."""
+    sc = scenes.test_scene(dims, seed=23, projection=JAC, iterations=11, n_obstacles=1,
+                           n_sources=2)
+    o = oracle_for(sc, 32)
+    o.step(sc["dt"], 2)
+    want = o.state()
+    for tile in (smokesim.TILE_64x32, smokesim.TILE_32x64):
+        for t in (2, 5, 6):
+            for zc in (1, dims[2]):
+                g = gpu_for(sc)
+                g.set_jacobi_plan(sweeps_per_launch=t, tile=tile, zc=zc)
+                for _ in range(2):
+                    g.step(sc["dt"], readback=False)
+                assert_bitwise_equal(g.state(), want, f"tile {tile} T={t} zc={zc}")
+                g.close()
