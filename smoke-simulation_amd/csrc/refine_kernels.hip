@@ -31,14 +31,6 @@ using dev::BY;
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ h2 ld_h2(const uint32_t* __restrict__ p, int w)
-{
-    return __builtin_bit_cast(h2, p[w]);
-}
-__device__ __forceinline__ void st_h2(uint32_t* __restrict__ p, int w, h2 v)
-{
-    p[w] = __builtin_bit_cast(uint32_t, v);
-}
 __device__ __forceinline__ float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 
 // Block reduction of an FP64 sum and an FP32 max (blocks of BX*BY threads), in
