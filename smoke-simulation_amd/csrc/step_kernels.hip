@@ -283,20 +283,25 @@ __global__ void k_project_ref_half(Grid g, float* __restrict__ u, float* __restr
     int x = dev::colour_x(t, y, z, colour);
     if (x > g.nx - 2) return;
     int i = g.idx(x, y, z);
+    // The six faces are loaded with the flags, not after them, so the launch
+    // waits for one round trip to memory instead of two (every interior cell's
+    // faces lie inside the arrays).
+    const float u0 = u[i], u1 = u[i + 1], v0 = v[i], v1 = v[i + g.px];
+    const float w0 = w[i], w1 = w[i + g.plane];
     uint8_t fl = flags[i];
     if (!(fl & F_ACTIVE)) return;  // solid (:365) or no fluid neighbour (:377)
     int sx0 = (fl >> 1) & 1, sx1 = (fl >> 2) & 1;
     int sy0 = (fl >> 3) & 1, sy1 = (fl >> 4) & 1;
     int sz0 = (fl >> 5) & 1, sz1 = (fl >> 6) & 1;
     int acc = sx0 + sx1 + sy0 + sy1 + sz0 + sz1;
-    float dv = -u[i] + u[i + 1] + -v[i] + v[i + g.px] + -w[i] + w[i + g.plane];
+    float dv = -u0 + u1 + -v0 + v1 + -w0 + w1;
     float p = (float)((double)(-dv / (float)acc) * omega);
-    u[i] -= p * (float)sx0;
-    u[i + 1] += p * (float)sx1;
-    v[i] -= p * (float)sy0;
-    v[i + g.px] += p * (float)sy1;
-    w[i] -= p * (float)sz0;
-    w[i + g.plane] += p * (float)sz1;
+    u[i] = u0 - p * (float)sx0;
+    u[i + 1] = u1 + p * (float)sx1;
+    v[i] = v0 - p * (float)sy0;
+    v[i + g.px] = v1 + p * (float)sy1;
+    w[i] = w0 - p * (float)sz0;
+    w[i + g.plane] = w1 + p * (float)sz1;
 }
 
 template <typename T>
