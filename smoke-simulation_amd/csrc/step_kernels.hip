@@ -361,12 +361,16 @@ __global__ void k_rbsor_half(Grid g, float* __restrict__ p, const float* __restr
     int x = dev::colour_x(t, y, z, colour);
     if (x > g.nx - 2) return;
     int i = g.idx(x, y, z);
+    // Loaded with the flags, not after them (one wait for memory instead of
+    // two); the neighbours are of the other colour, which this launch leaves
+    // alone, and every interior cell's neighbours lie inside the array.
+    const float pxm = p[i - 1], pxp = p[i + 1], pym = p[i - g.px], pyp = p[i + g.px];
+    const float pzm = p[i - g.plane], pzp = p[i + g.plane], pc = p[i], di = d[i];
     uint8_t fl = flags[i];
     if (!(fl & F_ACTIVE)) return;
-    const float G = dev::pressure_update(
-        dev::sum6(p[i - 1], p[i + 1], p[i - g.px], p[i + g.px], p[i - g.plane], p[i + g.plane]),
-        dev::inv_acc(__popc(fl & F_NBRS)), d[i]);
-    p[i] = p[i] + omega * (G - p[i]);
+    const float G = dev::pressure_update(dev::sum6(pxm, pxp, pym, pyp, pzm, pzp),
+                                         dev::inv_acc(__popc(fl & F_NBRS)), di);
+    p[i] = pc + omega * (G - pc);
 }
 
 // Warm start: zero the pressure of cells that are not active.
