@@ -271,37 +271,60 @@ __global__ void k_vort_apply(Grid g, T* __restrict__ u, T* __restrict__ v, T* __
 
 // divergence (smokeSimulation.cu:356-394): one red-black SOR half-sweep in
 // place on the face velocities.  Same-colour cells share no face, so the
-// in-place update is race-free.
-__global__ void k_project_ref_half(Grid g, float* __restrict__ u, float* __restrict__ v,
-                                   float* __restrict__ w, const uint8_t* __restrict__ flags,
-                                   int colour, double omega)
+// in-place update is race-free, and any assignment of the colour's cells to
+// threads gives the same bits.  The (column pair, row) index is flat over the
+// block's lanes (no idle lanes when a row holds fewer pairs than a wave), and
+// each thread takes kRefZ consecutive planes, loading every cell's six faces
+// and flags before it updates any (one wait for memory; every interior cell's
+// faces lie inside the arrays).
+constexpr int kRefZ = 2;
+
+__global__ __launch_bounds__(BX * BY) void k_project_ref_half(
+    Grid g, float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
+    const uint8_t* __restrict__ flags, int colour, double omega, int pairs)
 {
-    int t = blockIdx.x * BX + threadIdx.x;
-    int y = blockIdx.y * BY + threadIdx.y + 1;
-    int z = blockIdx.z + 1;
-    if (y > g.ny - 2 || z > g.nz - 2) return;
-    int x = dev::colour_x(t, y, z, colour);
-    if (x > g.nx - 2) return;
-    int i = g.idx(x, y, z);
-    // The six faces are loaded with the flags, not after them, so the launch
-    // waits for one round trip to memory instead of two (every interior cell's
-    // faces lie inside the arrays).
-    const float u0 = u[i], u1 = u[i + 1], v0 = v[i], v1 = v[i + g.px];
-    const float w0 = w[i], w1 = w[i + g.plane];
-    uint8_t fl = flags[i];
-    if (!(fl & F_ACTIVE)) return;  // solid (:365) or no fluid neighbour (:377)
-    int sx0 = (fl >> 1) & 1, sx1 = (fl >> 2) & 1;
-    int sy0 = (fl >> 3) & 1, sy1 = (fl >> 4) & 1;
-    int sz0 = (fl >> 5) & 1, sz1 = (fl >> 6) & 1;
-    int acc = sx0 + sx1 + sy0 + sy1 + sz0 + sz1;
-    float dv = -u0 + u1 + -v0 + v1 + -w0 + w1;
-    float p = (float)((double)(-dv / (float)acc) * omega);
-    u[i] = u0 - p * (float)sx0;
-    u[i + 1] = u1 + p * (float)sx1;
-    v[i] = v0 - p * (float)sy0;
-    v[i + g.px] = v1 + p * (float)sy1;
-    w[i] = w0 - p * (float)sz0;
-    w[i + g.plane] = w1 + p * (float)sz1;
+    const int f = (int)blockIdx.x * (BX * BY) + (int)threadIdx.y * BX + (int)threadIdx.x;
+    if (f >= pairs * (g.ny - 2)) return;
+    const int t = f % pairs, y = f / pairs + 1;
+    const int z0 = (int)blockIdx.y * kRefZ + 1;
+    float u0[kRefZ], u1[kRefZ], v0[kRefZ], v1[kRefZ], w0[kRefZ], w1[kRefZ];
+    uint8_t fl[kRefZ];
+    int ic[kRefZ];
+#pragma unroll
+    for (int j = 0; j < kRefZ; ++j) {
+        const int z = z0 + j, x = dev::colour_x(t, y, z, colour);
+        fl[j] = 0;
+        ic[j] = 0;
+        u0[j] = u1[j] = v0[j] = v1[j] = w0[j] = w1[j] = 0.0f;
+        if (z > g.nz - 2 || x > g.nx - 2) continue;
+        const int i = g.idx(x, y, z);
+        ic[j] = i;
+        u0[j] = u[i];
+        u1[j] = u[i + 1];
+        v0[j] = v[i];
+        v1[j] = v[i + g.px];
+        w0[j] = w[i];
+        w1[j] = w[i + g.plane];
+        fl[j] = flags[i];
+    }
+#pragma unroll
+    for (int j = 0; j < kRefZ; ++j) {
+        const uint8_t c = fl[j];
+        if (!(c & F_ACTIVE)) continue;  // solid (:365), no fluid neighbour (:377), or no cell
+        const int i = ic[j];
+        int sx0 = (c >> 1) & 1, sx1 = (c >> 2) & 1;
+        int sy0 = (c >> 3) & 1, sy1 = (c >> 4) & 1;
+        int sz0 = (c >> 5) & 1, sz1 = (c >> 6) & 1;
+        int acc = sx0 + sx1 + sy0 + sy1 + sz0 + sz1;
+        float dv = -u0[j] + u1[j] + -v0[j] + v1[j] + -w0[j] + w1[j];
+        float p = (float)((double)(-dv / (float)acc) * omega);
+        u[i] = u0[j] - p * (float)sx0;
+        u[i + 1] = u1[j] + p * (float)sx1;
+        v[i] = v0[j] - p * (float)sy0;
+        v[i + g.px] = v1[j] + p * (float)sy1;
+        w[i] = w0[j] - p * (float)sz0;
+        w[i + g.plane] = w1[j] + p * (float)sz1;
+    }
 }
 
 template <typename T>
@@ -623,9 +646,12 @@ hipError_t vorticity(const Grid& g, float* u, float* v, float* w, const uint8_t*
 hipError_t project_reference_half(const Grid& g, float* u, float* v, float* w,
                                   const uint8_t* flags, int colour, double omega, hipStream_t st)
 {
-    int tx = (g.nx - 2 + 1) / 2;
-    hipLaunchKernelGGL(k_project_ref_half, grid_for(tx, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g,
-                       u, v, w, flags, colour, omega);
+    const int pairs = (g.nx - 2 + 1) / 2;  // cells of one colour per row, at most
+    const long long items = (long long)pairs * (g.ny - 2);
+    const dim3 grid((unsigned)((items + BX * BY - 1) / (BX * BY)),
+                    (unsigned)((g.nz - 2 + kRefZ - 1) / kRefZ));
+    hipLaunchKernelGGL(k_project_ref_half, grid, dim3(BX, BY), 0, st, g, u, v, w, flags, colour,
+                       omega, pairs);
     return hipGetLastError();
 }
 
