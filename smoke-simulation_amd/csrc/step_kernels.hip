@@ -374,26 +374,51 @@ __global__ __launch_bounds__(TX * TY / R, 8 / R) void k_jacobi_fused(
 }
 
 // One red-black SOR half-sweep of the pressure form, in place.
-__global__ void k_rbsor_half(Grid g, float* __restrict__ p, const float* __restrict__ d,
-                             const uint8_t* __restrict__ flags, int colour, float omega)
+__global__ __launch_bounds__(BX * BY) void k_rbsor_half(Grid g, float* __restrict__ p,
+                                                    const float* __restrict__ d,
+                                                    const uint8_t* __restrict__ flags, int colour,
+                                                    float omega, int pairs)
 {
-    int t = blockIdx.x * BX + threadIdx.x;
-    int y = blockIdx.y * BY + threadIdx.y + 1;
-    int z = blockIdx.z + 1;
-    if (y > g.ny - 2 || z > g.nz - 2) return;
-    int x = dev::colour_x(t, y, z, colour);
-    if (x > g.nx - 2) return;
-    int i = g.idx(x, y, z);
-    // Loaded with the flags, not after them (one wait for memory instead of
-    // two); the neighbours are of the other colour, which this launch leaves
-    // alone, and every interior cell's neighbours lie inside the array.
-    const float pxm = p[i - 1], pxp = p[i + 1], pym = p[i - g.px], pyp = p[i + g.px];
-    const float pzm = p[i - g.plane], pzp = p[i + g.plane], pc = p[i], di = d[i];
-    uint8_t fl = flags[i];
-    if (!(fl & F_ACTIVE)) return;
-    const float G = dev::pressure_update(dev::sum6(pxm, pxp, pym, pyp, pzm, pzp),
-                                         dev::inv_acc(__popc(fl & F_NBRS)), di);
-    p[i] = pc + omega * (G - pc);
+    // The same thread layout as k_project_ref_half: a flat (column pair, row)
+    // index and kRefZ consecutive planes per thread, every load issued before
+    // any update.  The neighbours are of the other colour, which this launch
+    // leaves alone, and every interior cell's neighbours lie inside the array.
+    const int f = (int)blockIdx.x * (BX * BY) + (int)threadIdx.y * BX + (int)threadIdx.x;
+    if (f >= pairs * (g.ny - 2)) return;
+    const int t = f % pairs, y = f / pairs + 1;
+    const int z0 = (int)blockIdx.y * kRefZ + 1;
+    float nb[kRefZ][6], pc[kRefZ], di[kRefZ];
+    uint8_t fl[kRefZ];
+    int ic[kRefZ];
+#pragma unroll
+    for (int j = 0; j < kRefZ; ++j) {
+        const int z = z0 + j, x = dev::colour_x(t, y, z, colour);
+        fl[j] = 0;
+        ic[j] = 0;
+        pc[j] = di[j] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nb[j][k] = 0.0f;
+        if (z > g.nz - 2 || x > g.nx - 2) continue;
+        const int i = g.idx(x, y, z);
+        ic[j] = i;
+        nb[j][0] = p[i - 1];
+        nb[j][1] = p[i + 1];
+        nb[j][2] = p[i - g.px];
+        nb[j][3] = p[i + g.px];
+        nb[j][4] = p[i - g.plane];
+        nb[j][5] = p[i + g.plane];
+        pc[j] = p[i];
+        di[j] = d[i];
+        fl[j] = flags[i];
+    }
+#pragma unroll
+    for (int j = 0; j < kRefZ; ++j) {
+        if (!(fl[j] & F_ACTIVE)) continue;
+        const float G = dev::pressure_update(
+            dev::sum6(nb[j][0], nb[j][1], nb[j][2], nb[j][3], nb[j][4], nb[j][5]),
+            dev::inv_acc(__popc(fl[j] & F_NBRS)), di[j]);
+        p[ic[j]] = pc[j] + omega * (G - pc[j]);
+    }
 }
 
 // Warm start: zero the pressure of cells that are not active.
@@ -739,9 +764,12 @@ hipError_t jacobi_fused(const Grid& g, const float* pin, float* pout, const floa
 hipError_t rbsor_half(const Grid& g, float* p, const float* d, const uint8_t* flags, int colour,
                       float omega, hipStream_t st)
 {
-    int tx = (g.nx - 2 + 1) / 2;
-    hipLaunchKernelGGL(k_rbsor_half, grid_for(tx, g.ny - 2, g.nz - 2), dim3(BX, BY), 0, st, g, p, d,
-                       flags, colour, omega);
+    const int pairs = (g.nx - 2 + 1) / 2;
+    const long long items = (long long)pairs * (g.ny - 2);
+    const dim3 grid((unsigned)((items + BX * BY - 1) / (BX * BY)),
+                    (unsigned)((g.nz - 2 + kRefZ - 1) / kRefZ));
+    hipLaunchKernelGGL(k_rbsor_half, grid, dim3(BX, BY), 0, st, g, p, d, flags, colour, omega,
+                       pairs);
     return hipGetLastError();
 }
 
